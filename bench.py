@@ -1,0 +1,165 @@
+"""Benchmark of the forward Gaussian ray-tracing path on one MI355X.
+
+Workload (one step = one forward frame, traceRays of the whole image):
+  1,000,000 Gaussians, seeded 64-cluster mixture (centres uniform in a 20^3 box,
+  offsets N(0, 0.8)), log-scales N(-4, 0.6), opacity logit N(0.5, 1.5), SH degree 3;
+  one 1920x1080 pinhole camera, hFov 50 deg, at a seeded position inside the
+  volume looking at the origin; alpha_min 0.01, k (hit_array_size) 16,
+  T_min 0.001, t_min 1e-3, t_max 1e5, FP32.
+Inputs are resident in HBM before the timed region; the output image is
+written to device memory.  value = rays / time (Mrays/s).
+
+roofline: algorithmic HBM bytes per frame
+  B_fwd = 16 R + 44 H_eval + 192 H_comp
+(R rays, H_eval evaluated k-buffer entries, H_comp composited entries: the
+RGBA write, the 44-B mu/scale/rotation/opacity read per evaluated hit and
+the 192-B SH read per composited hit), divided by the kernel's average
+duration measured with HIP events on the launch stream.
+
+cpu_baseline: the oracle (the project's C restatement of the reference path,
+BVH variant, all host threads up to 16) on a seeded 1/64 sample of the rays.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "webgpu-gaussiantracer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
+
+
+def camera_c2(seed=1):
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pos = rng.uniform(-10.0, 10.0, 3)
+    return [float(x) for x in pos], [0.0, 0.0, 0.0]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--n", type=int, default=1000000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="time 1/N of the rays on the CPU")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(rays shard with no exchange; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import gtrace as gt
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    W, H, K = args.width, args.height, args.k
+    R = W * H
+
+    t0 = time.time()
+    raw = gt.generate_raw(n=args.n, seed=0, config="c2")
+    t_gen = time.time() - t0
+    t0 = time.time()
+    scene = gt.Scene(raw, alpha_min=0.01)
+    t_build = time.time() - t0
+    info = scene.info()
+    pos, tgt = camera_c2()
+    tw = gt.camera_look_at(pos, tgt)
+    u = gt.make_uniforms(tw, W, H, hfov_deg=50.0, t_min=1e-3, t_max=1e5, T_min=1e-3, alpha_min=0.01, k=K,
+                         sh_degree_max=3)
+
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    stats = torch.zeros(8, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    # counters for the algorithmic byte count (separate, untimed launch)
+    scene.render_device(u, W, H, rgba.data_ptr(), sp, stats.data_ptr())
+    torch.cuda.synchronize()
+    st = dict(zip(gt.STAT_NAMES, [int(x) for x in stats.cpu().tolist()]))
+
+    for _ in range(args.warmup):
+        scene.render_device(u, W, H, rgba.data_ptr(), sp)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        scene.render_device(u, W, H, rgba.data_ptr(), sp)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    mrays = R / (ms_step * 1e-3) / 1e6
+
+    B_fwd = 16 * R + 44 * st["h_eval"] + 192 * st["h_comp"]
+    achieved = B_fwd / (ms_step * 1e-3) / 1e9
+    img = rgba.cpu().numpy()
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, scene seed 0, camera seed 1)",
+        "config": {
+            "workload": f"forward render, {args.n} Gaussians 64-cluster mixture SH3, {W}x{H}, hFov 50, "
+                        f"alpha_min 0.01, k {K}, T_min 0.001",
+            "gaussians": args.n, "width": W, "height": H, "k": K, "camera_pos": [round(x, 4) for x in pos],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16,0,false,false>",
+        },
+        "stats": st,
+        "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
+        "scene": {"n_proxies": info["n_proxies"], "n_nodes": info["n_nodes"], "bvh_depth": info["bvh_depth"],
+                  "max_stack": info["max_stack"], "device_bytes": info["device_bytes"],
+                  "host_build_s": round(t_build, 3), "generate_s": round(t_gen, 3)},
+        "image_mean": float(img[..., :3].mean()),
+    }
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        o = orc.OracleScene(raw, 0.01)
+        o.prebuild_bvh()
+        p = orc.params_from_uniforms(u, W, H)
+        rng = np.random.Generator(np.random.PCG64(2))
+        n_s = max(1, R // args.cpu_sample)
+        rays = np.sort(rng.choice(R, n_s, replace=False)).astype(np.uint32)
+        c0 = time.perf_counter()
+        rr = orc.render(o, p, threads=threads, ray_list=rays)
+        cs = time.perf_counter() - c0
+        out["cpu_baseline"] = {
+            "value": round(n_s / cs / 1e6, 5), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{n_s} seeded random rays (1/{args.cpu_sample} of the frame), oracle BVH variant, "
+                      f"{cs:.2f} s",
+            "h_eval_per_ray": rr["stats"]["h_eval"] / n_s,
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * gt.h - C-ABI of the MI355X Gaussian ray tracer (libgt.so).
+ *
+ * Drop-in for the reference's ray-tracing path: everything the WebRTX
+ * extension to WebGPU plus the app's frame loop do for one frame
+ *   createRayTracingAccelerationContainer + hostBuildRayTracingAccelerationContainer
+ *       (webrtx/src/patch.ts:211-219, src/main.ts:30-111)      -> gt_scene_create_*
+ *   uniform block update (src/main.ts:520-534)                 -> gt_uniforms_init
+ *   beginRayTracingPass().traceRays(device, sbt, W, H)
+ *       (webrtx/src/pass_encoder.ts:35-65, src/main.ts:563-593) -> gt_render
+ * behind plain C types: pointers, sizes, status codes.  No C++ exception
+ * crosses this boundary; every entry point returns GT_OK (0) or a negative
+ * GT_E* code, and gt_last_error() gives the calling thread's message
+ * (the reference throws JS strings, pass_encoder.ts:27-57).
+ *
+ * Threading: calls on one scene handle are serialised by the caller (as the
+ * reference's single JS thread does); different handles may be used from
+ * different threads.  Buffers are caller-owned.  With GT_BUF_DEVICE the
+ * pointers are HIP device pointers and the call is asynchronous on the given
+ * stream; otherwise they are host pointers and the call is synchronous.
+ */
+#ifndef GT_H
+#define GT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_OK 0
+#define GT_E_INVALID (-1)   /* bad argument (null pointer, size, range)        */
+#define GT_E_HIP (-2)       /* HIP runtime error                               */
+#define GT_E_NOMEM (-3)     /* allocation failed                               */
+#define GT_E_IO (-4)        /* file could not be read / parsed                 */
+#define GT_E_LIMIT (-5)     /* exceeds a documented limit (k, rounds, stack)   */
+
+#define GT_RAW_FLOATS 59    /* PLY-raw record: x y z f_dc_0..2 f_rest_0..44 opacity scale_0..2 rot_0..3 */
+#define GT_PACKED_BYTES 320 /* GaussianParticle record, src/GS_shader.ts:81-87, src/GS_ply.ts:199-205 */
+#define GT_MAX_K 32         /* hit_array_size limit (the reference compiles MAX_K = k, GS_shader.ts:377) */
+#define GT_SENTINEL_ID 0xffffffffu /* empty k-buffer slot, GS_shader.ts:239 */
+
+/* Buffer flags */
+#define GT_BUF_HOST 0
+#define GT_BUF_DEVICE 1
+
+typedef struct gt_scene gt_scene;
+
+/* 112-byte uniform block, byte for byte as src/GS_shader.ts:96-108 and the
+ * packing of src/main.ts:516-534 (fview/uview indices in comments). */
+typedef struct gt_uniforms {
+  float to_world[16];          /* 0   camera matrix, column-major (camera.ts)  [0..15] */
+  float camera_aspect[2];      /* 64  tan(hFov/2), tan(hFov/2)*H/W            [16,17] */
+  float t_min;                 /* 72  [18] */
+  float t_max;                 /* 76  [19] */
+  float T_min;                 /* 80  [20] */
+  float alpha_min;             /* 84  [21] */
+  uint32_t k;                  /* 88  hit_array_size [22] */
+  uint32_t sh_degree_max;      /* 92  [23] */
+  uint32_t accumulated_frames; /* 96  [24] (AA is compiled out in the reference) */
+  uint32_t early_stop;         /* 100 0xffffffff = never [25] */
+  uint32_t debug_mode;         /* 104 0 render, 1 closest proxy, 2 hit count [26] */
+  uint32_t _pad;               /* 108 */
+} gt_uniforms;
+
+/* Fills the block as upadateUniformData does (src/main.ts:520-534):
+ * aspect = (f32(tan(0.5*hFov*pi/180)), f32(aspect.x * H / W)),
+ * early_stop <= 0 -> 0xffffffff. */
+void gt_uniforms_init(gt_uniforms* u, const float to_world[16], double hfov_deg, uint32_t width,
+                      uint32_t height, float t_min, float t_max, float T_min, float alpha_min,
+                      uint32_t k, uint32_t sh_degree_max, int32_t early_stop, uint32_t debug_mode);
+
+/* WASDCamera matrix (src/camera.ts:126-140, 153-204) for a camera at pos
+ * looking at target after its first update(): rotY(yaw)*rotX(pitch) with
+ * yaw = atan2(b.x, b.z), pitch = -asin(b.y), b = normalize(pos - target). */
+void gt_camera_look_at(const float pos[3], const float target[3], float to_world[16]);
+
+/* ---- scene (acceleration container) ---- */
+
+/* From PLY-raw parameters (n x 59 floats): applies the activations of
+ * src/GS_ply.ts:225-235 in double, packs 320-B records, filters proxies by
+ * opacity > alpha_min and builds their transforms as src/main.ts:73-94 does,
+ * then builds and uploads the BVH.  alpha_min is the JS double of
+ * main.ts:463 (the uniform's f32 alpha_min is separate). */
+int gt_scene_create_raw(const float* raw, uint32_t n, double alpha_min, gt_scene** out);
+
+/* From already packed 320-B records (n x 320 bytes, the reference's
+ * gsData.gaussiansBuffer).  Proxy transforms are then built from the f32
+ * activated values, since the JS doubles are not part of the buffer. */
+int gt_scene_create_packed(const void* records, uint32_t n, double alpha_min, gt_scene** out);
+
+/* Re-filter and rebuild for a new alpha_min (the reference rebuilds the TLAS). */
+int gt_scene_set_alpha_min(gt_scene* s, double alpha_min);
+
+void gt_scene_destroy(gt_scene* s);
+
+typedef struct gt_scene_info {
+  uint32_t n_gaussians;
+  uint32_t n_proxies;       /* opacity > alpha_min */
+  uint32_t n_nodes;         /* wide BVH nodes */
+  uint32_t max_stack;       /* traversal stack entries the BVH can need */
+  uint32_t bvh_depth;
+  uint32_t _pad;
+  uint64_t device_bytes;    /* HBM held by the scene */
+  double build_ms;          /* host build time (activations, proxies, BVH) */
+} gt_scene_info;
+int gt_scene_get_info(const gt_scene* s, gt_scene_info* info);
+
+/* Host copies of what the scene built (for tests): packed records
+ * (n x 80 floats), proxy ids / object-to-world / world-to-object
+ * (n_proxies, n_proxies x 12 floats each, column-major 4x3).  Any may be NULL. */
+int gt_scene_get_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w, float* w2o);
+
+/* Host stage of gt_scene_create_raw, callable without a GPU: activations,
+ * 320-B packing and proxy construction (src/GS_ply.ts:225-252,
+ * src/main.ts:73-94).  packed: n x 80 floats; proxy_ids: n; o2w, w2o:
+ * n x 12 floats; *n_proxies receives how many were written. */
+int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, float* packed, uint32_t* proxy_ids, float* o2w,
+                    float* w2o, uint32_t* n_proxies);
+
+/* ---- render (traceRays) ---- */
+
+/* Per-frame outputs.  rgba: W*H float4, row-major y*W+x, A = 1
+ * (GS_shader.ts:300-301).  Optional parity outputs (NULL to skip):
+ *   hit_ids  W*H*max_rounds*k Gaussian ids per ray and round, sentinel-filled
+ *   accept   W*H*max_rounds   bit i set = entry i was composited (alpha > alpha_min)
+ *   n_rounds W*H              rounds run
+ *   T        W*H              final transmittance
+ *   stats    GT_NSTATS u64    accumulated counters (device pointer with GT_BUF_DEVICE) */
+#define GT_NSTATS 8
+/* stats: 0 rays, 1 rounds, 2 H_eval (evaluated k-buffer entries), 3 H_comp
+ * (composited), 4 proxies tested, 5 front-face hits, 6 BVH nodes visited, 7 rounds beyond max_rounds */
+typedef struct gt_frame {
+  float* rgba;
+  uint32_t* hit_ids;
+  uint32_t* accept;
+  uint32_t* n_rounds;
+  float* T;
+  uint64_t* stats;
+  uint32_t max_rounds;
+  uint32_t _pad;
+} gt_frame;
+
+/* traceRays(W, H) of the reference with the uniform block u.  Unlike the
+ * reference (pass_encoder.ts:52-56) W and H need not be multiples of 8;
+ * limits: 1..16384 each, 1 <= k <= 32, t_min >= 0, sh_degree_max <= 3. */
+int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
+              int buf_flags, void* hip_stream);
+
+/* ---- synthetic scenes (seeded SplitMix64 + Box-Muller) ---- */
+typedef struct gt_gen_params {
+  uint32_t n;
+  uint32_t n_clusters;      /* 0: means uniform in the box; else mixture */
+  float box_half;           /* means / cluster centres uniform in [-box_half, box_half]^3 */
+  float cluster_sigma;      /* member offsets N(0, sigma) per axis */
+  float log_scale_mean, log_scale_std;
+  float logit_mean, logit_std;
+  float dc_std, rest_std;
+  uint64_t seed;
+} gt_gen_params;
+int gt_generate_scene(const gt_gen_params* g, float* raw /* n x 59 */);
+
+/* Reads a binary-little-endian 3DGS .ply (float/uchar properties, SH degree
+ * 0..3) into PLY-raw records (src/GS_ply.ts:58-132, 171-196).  Missing SH
+ * coefficients are zero.  *raw is malloc'ed; release with gt_free. */
+int gt_ply_read(const char* path, float** raw, uint32_t* n, uint32_t* sh_degree);
+void gt_free(void* p);
+
+const char* gt_last_error(void);
+const char* gt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GT_H */

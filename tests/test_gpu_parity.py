@@ -1,0 +1,146 @@
+"""GPU parity of the fused HIP forward kernel against the CPU oracle.
+
+Bar: per-ray, per-round hit lists (Gaussian ids), accept masks, round counts
+and final T bit-exact; pixels within 1e-4 absolute per channel of the FP64
+re-evaluation of the same hit lists; alpha channel exactly 1.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import check_parity, scene_pair, uniforms
+import gtrace as gt
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_c1_config_full():
+    """20k Gaussians uniform in a 4^3 box, 256x256, hFov 60, camera at (0,0,8),
+    alpha_min 0.1, k 6, T_min 0.03 (the reference's CPU-runnable case)."""
+    _print("case c1 full 256x256 k6")
+    raw = gt.generate_raw(seed=0, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    u = uniforms([0, 0, 8], [0, 0, 0], 256, 256, hfov_deg=60, alpha_min=0.1, k=6, T_min=0.03)
+    g, r = check_parity(s, o, u, 256, 256, label="c1")
+    assert g["stats"]["h_comp"] > 1000
+
+
+def test_cluster_scene_camera_inside():
+    """Clustered scene (C2 generator at 1/10 size), camera inside the volume, k=16."""
+    _print("case clusters 100k 160x90 k16")
+    raw = gt.generate_raw(n=100000, seed=0, config="c2", n_clusters=8)
+    s, o = scene_pair(raw, 0.01)
+    c0 = raw[0::8, :3].mean(axis=0)  # centre of cluster 0 (Gaussian i is in cluster i mod 8)
+    u = uniforms(list(c0 + np.array([0.4, -0.5, 2.5])), list(c0), 160, 90, hfov_deg=50, alpha_min=0.01, k=16,
+                 T_min=0.001)
+    g, r = check_parity(s, o, u, 160, 90, max_rounds=64, label="clusters")
+    assert g["stats"]["rounds"] > g["stats"]["rays"]  # multi-round rays exist
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 13, 16, 24, 32])
+def test_k_sweep(k):
+    _print(f"case k sweep k={k}")
+    raw = gt.generate_raw(n=20000, seed=3, config="c4")
+    s, o = scene_pair(raw, 0.05)
+    u = uniforms([0, 0, 3.0], [0, 0, 0], 48, 40, hfov_deg=50, alpha_min=0.05, k=k, T_min=0.01)
+    check_parity(s, o, u, 48, 40, max_rounds=128, label=f"k{k}")
+
+
+@pytest.mark.parametrize("W,H", [(37, 23), (1, 1), (8, 1), (1, 9), (65, 7)])
+def test_odd_sizes(W, H):
+    _print(f"case size {W}x{H}")
+    raw = gt.generate_raw(n=5000, seed=4, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    u = uniforms([0.3, 0.2, 5.0], [0, 0, 0], W, H, hfov_deg=40, alpha_min=0.1, k=6, T_min=0.03)
+    check_parity(s, o, u, W, H, label=f"{W}x{H}")
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_sh_degrees(deg):
+    _print(f"case sh degree {deg}")
+    raw = gt.generate_raw(n=5000, seed=5, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    u = uniforms([0, 1, 6], [0, 0, 0], 32, 32, alpha_min=0.1, k=6, sh_degree_max=deg)
+    check_parity(s, o, u, 32, 32, label=f"deg{deg}")
+
+
+@pytest.mark.parametrize("early", [1, 2])
+def test_early_stop(early):
+    _print(f"case early stop {early}")
+    raw = gt.generate_raw(n=20000, seed=6, config="c4")
+    s, o = scene_pair(raw, 0.05)
+    u = uniforms([0, 0, 3.0], [0, 0, 0], 32, 32, alpha_min=0.05, k=2, T_min=0.01, early_stop=early)
+    g, _ = check_parity(s, o, u, 32, 32, label=f"early{early}")
+    assert g["n_rounds"].max() <= early
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_debug_modes(mode):
+    _print(f"case debug mode {mode}")
+    raw = gt.generate_raw(n=20000, seed=7, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    u = uniforms([0, 0, 8], [0, 0, 0], 64, 64, alpha_min=0.1, k=6, debug_mode=mode)
+    W = H = 64
+    g = s.render(u, W, H, max_rounds=1)
+    import oracle as orc
+    p = orc.params_from_uniforms(u, W, H)
+    r = orc.render(o, p, max_rounds=1)
+    np.testing.assert_array_equal(g["hit_ids"], r["hit_ids"])
+    np.testing.assert_allclose(g["rgba"].reshape(-1, 4), r["rgba"], atol=1e-5, rtol=0)
+
+
+def test_no_proxies_is_black():
+    """No Gaussian above alpha_min: black image, A = 1, one empty round
+    (the reference's build_tlas would panic here, lib.rs:518-519)."""
+    _print("case empty scene")
+    raw = gt.generate_raw(n=1000, seed=8, config="c1", logit_mean=-12.0, logit_std=0.1)
+    s, o = scene_pair(raw, 0.1)
+    assert s.info()["n_proxies"] == 0
+    u = uniforms([0, 0, 8], [0, 0, 0], 16, 16, alpha_min=0.1, k=6)
+    g, _ = check_parity(s, o, u, 16, 16, label="empty")
+    assert np.all(g["rgba"][..., :3] == 0) and np.all(g["rgba"][..., 3] == 1)
+
+
+def test_edge_hit_same_gaussian_twice():
+    """A ray along +z through an axis-aligned Gaussian crosses the edge shared by
+    two icosahedron faces: with inclusive barycentrics both faces report the hit,
+    so the Gaussian occupies two k-buffer slots (reference behaviour)."""
+    _print("case edge hit")
+    raw = np.zeros((1, 59), np.float32)
+    raw[0, 51] = 3.0          # opacity logit
+    raw[0, 52:55] = np.log(0.5)
+    raw[0, 55] = 1.0          # identity quaternion
+    raw[0, 3:6] = 1.0
+    s, o = scene_pair(raw, 0.01)
+    u = uniforms([0, 0, 5.0], [0, 0, 0], 1, 1, alpha_min=0.01, k=4, T_min=1e-6)
+    g, r = check_parity(s, o, u, 1, 1, label="edge")
+    ids = g["hit_ids"][0, 0]
+    assert list(ids[:2]) == [0, 0], ids
+
+
+def test_pc_short_ply():
+    """The reference's only data file: 100 real 3DGS Gaussians."""
+    _print("case pc_short.ply")
+    raw, deg = gt.read_ply(os.path.join(ROOT, "tests", "golden", "pc_short.ply"))
+    assert deg == 3 and raw.shape[0] == 100
+    c = raw[:, :3].mean(axis=0)
+    s, o = scene_pair(raw, 0.01)
+    u = uniforms(list(c + np.array([0.0, 0.0, 3.0])), list(c), 64, 64, alpha_min=0.01, k=4, T_min=0.001)
+    check_parity(s, o, u, 64, 64, label="pc_short")
+
+
+def test_determinism_same_frame_twice():
+    _print("case determinism")
+    raw = gt.generate_raw(n=50000, seed=9, config="c2")
+    s = gt.Scene(raw, alpha_min=0.01)
+    u = uniforms([0.5, 0.5, 2.0], [0, 0, 0], 96, 64, hfov_deg=50, alpha_min=0.01, k=16, T_min=0.001)
+    a = s.render(u, 96, 64, want_T=True)
+    b = s.render(u, 96, 64, want_T=True)
+    assert a["rgba"].tobytes() == b["rgba"].tobytes() and a["T"].tobytes() == b["T"].tobytes()

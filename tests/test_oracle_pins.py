@@ -1,0 +1,422 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no golden vectors, known-answer tests or fixtures beyond
+data/pc_short.ply (its only unit test, webrtx/bvh/src/lib.rs:605-749, has no
+assertion).  Each part of the oracle is therefore checked against an
+independent derivation: closed forms, FP64 re-derivations with textbook
+formulas (Moller-Trumbore, real spherical harmonics from scipy, the
+line-quadric minimum), brute force on tiny inputs and invariants.
+test_mutation_detected plants plausible mistakes in the oracle (through its
+mutation switch) and requires the matching pin to fail.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+import gtrace as gt
+import oracle as orc
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------- helpers
+def quat_to_R(q):
+    """Standard rotation matrix of a unit quaternion (w, x, y, z)."""
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def ico_geometry():
+    """Icosahedron of src/main.ts:12-25 from its defining formulas, in double
+    over the f32-rounded coordinates the reference stores."""
+    r = (3 + math.sqrt(5.0)) / (2 * math.sqrt(3.0))
+    s = float(F32(1 / r))
+    t = float(F32((1.0 + math.sqrt(5.0)) / (2.0 * r)))
+    V = np.array([[-s, t, 0], [s, t, 0], [-s, -t, 0], [s, -t, 0], [0, -s, t], [0, s, t], [0, -s, -t], [0, s, -t],
+                  [t, 0, -s], [t, 0, s], [-t, 0, -s], [-t, 0, s]], np.float64)
+    I = np.array([0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+                  3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1]).reshape(20, 3)
+    return V, I
+
+
+def packed_record(mu, log_scale, quat, logit, dc=(0, 0, 0), rest=None):
+    raw = np.zeros((1, 59), F32)
+    raw[0, :3] = mu
+    raw[0, 3:6] = dc
+    if rest is not None:
+        raw[0, 6:51] = rest
+    raw[0, 51] = logit
+    raw[0, 52:55] = log_scale
+    raw[0, 55:59] = quat
+    return raw
+
+
+def mt64(o, d, p0, p1, p2):
+    """Moller-Trumbore in double: (t, u, v) of the line o + t d with the triangle."""
+    e1, e2 = p1 - p0, p2 - p0
+    h = np.cross(d, e2)
+    a = e1 @ h
+    f = 1.0 / a
+    s = o - p0
+    u = f * (s @ h)
+    q = np.cross(s, e1)
+    v = f * (d @ q)
+    t = f * (e2 @ q)
+    return t, u, v
+
+
+# ---------------------------------------------------------------- pins
+def check_expf():
+    xs = np.concatenate([np.linspace(-85.9, 88.0, 120001), -np.logspace(-9, 1.9, 1500),
+                         np.logspace(-9, 1.9, 1500), [0.0, -1e-30, 1e-30]]).astype(F32)
+    got = np.array([orc.expf(float(x)) for x in xs], np.float64)
+    ref = np.exp(xs.astype(np.float64))
+    ulp = np.spacing(ref.astype(F32)).astype(np.float64)
+    err = np.abs(got - ref) / ulp
+    assert err.max() <= 2.0, f"expf error {err.max()} ulp at x={xs[err.argmax()]}"
+    assert orc.expf(-86.5) == 0.0 and orc.expf(-1e4) == 0.0 and orc.expf(float("nan")) == 0.0
+    assert math.isinf(orc.expf(89.0))
+
+
+def check_icosahedron():
+    V, I = ico_geometry()
+    for f in range(20):
+        p0, p1, p2 = V[I[f]]
+        n = np.cross(p0 - p2, p1 - p0)  # cross(e1, e0) of intersect.glsl:78-80
+        nh = n / np.linalg.norm(n)
+        assert abs(nh @ p0 - 1.0) < 1e-6, "face plane at distance 1 (inradius 1)"
+        assert nh @ (p0 + p1 + p2) > 0, "faces are outward with n = cross(e1, e0)"
+    assert np.allclose(np.linalg.norm(V, axis=1), 1.2584085723648188, atol=1e-6)
+
+
+def check_ray_dir():
+    pos, tgt = [3.0, 2.0, 7.0], [0.2, -0.1, 0.3]
+    tw = gt.camera_look_at(pos, tgt)
+    W, H = 33, 17
+    u = gt.make_uniforms(tw, W, H, hfov_deg=50)
+    p = orc.params_from_uniforms(u, W, H)
+    M = tw.astype(np.float64).reshape(4, 4)  # row c = column c of the column-major matrix
+    X, Y, Z = -M[0, :3], M[1, :3], -M[2, :3]
+    ax = math.tan(0.5 * 50 * math.pi / 180)
+    ay = float(F32(ax)) * H / W
+    for px, py in [(0, 0), (32, 16), (5, 11), (16, 8), (31, 2)]:
+        ndc = np.array([(px + 0.5) / W * 2 - 1, (py + 0.5) / H * 2 - 1])
+        v = ndc[0] * ax * X + ndc[1] * ay * Y + Z
+        v /= np.linalg.norm(v)
+        d = orc.ray_dir(p, px, py)
+        assert np.abs(d - v).max() < 2e-6, (px, py, d, v)
+    # centre pixel looks along -back = -(pos - target)/|pos - target|
+    b = np.array(pos) - np.array(tgt)
+    assert np.abs(orc.ray_dir(p, 16, 8) + b / np.linalg.norm(b)).max() < 2e-6
+    # right/up of the camera.ts matrix: x grows to -col0 ... the image is rotated 180 deg vs y-down
+    d00, dWH = orc.ray_dir(p, 0, 0), orc.ray_dir(p, W - 1, H - 1)
+    assert np.abs((d00 + dWH) / 2 - (d00 + dWH) @ Z / 2 * Z / (Z @ Z)).max() < 1e-6
+
+
+def check_proxy_tangent():
+    """Every proxy face is tangent to the Mahalanobis sphere of radius
+    s = sqrt(2 ln(opacity / alpha_min)) around the mean (main.ts:84-87),
+    with Sigma = R S^2 R^T, R the standard rotation of the quaternion."""
+    raw = gt.generate_raw(n=200, seed=11, config="c1")
+    alpha = 0.05
+    packed, act = orc.activate(raw)
+    ids, o2w, w2o = orc.build_proxies(raw, act, alpha)
+    assert len(ids) > 100
+    V, I = ico_geometry()
+    for j, gid in enumerate(ids[:60]):
+        op, sc, q = act[gid, 0], act[gid, 1:4], act[gid, 4:8]
+        s = math.sqrt(2 * math.log(op / alpha))
+        R = quat_to_R(q)
+        Sigma = R @ np.diag(sc ** 2) @ R.T
+        M = o2w[j].astype(np.float64).reshape(4, 3).T  # 3x4 [R S s | mu]
+        A, t = M[:, :3], M[:, 3]
+        assert np.allclose(t, raw[gid, :3])
+        Ainv_T = np.linalg.inv(A).T
+        for f in range(20):
+            p0, p1, p2 = V[I[f]]
+            n = np.cross(p0 - p2, p1 - p0)
+            nh = n / np.linalg.norm(n)
+            a = Ainv_T @ nh
+            dm = 1.0 / math.sqrt(a @ Sigma @ a)
+            assert abs(dm / s - 1) < 2e-5, (gid, f, dm, s)
+        Wm = w2o[j].astype(np.float64).reshape(4, 3).T
+        comp = Wm[:, :3] @ np.vstack([A.T, t]).T[:, :3]
+        assert np.allclose(comp, np.eye(3), atol=2e-5)
+        assert np.allclose(Wm[:, :3] @ t + Wm[:, 3], 0, atol=2e-5 * (1 + np.abs(Wm[:, 3]).max()))
+
+
+def check_tri_vs_mt():
+    rng = np.random.default_rng(5)
+    V, I = ico_geometry()
+    agree = 0
+    for _ in range(3000):
+        o = rng.normal(size=3) * 3
+        tgt = rng.normal(size=3) * 0.6
+        d = tgt - o
+        d /= np.linalg.norm(d)
+        o32, d32 = o.astype(F32), d.astype(F32)
+        for f in range(20):
+            p0, p1, p2 = V[I[f]]
+            t, u, v = mt64(o32.astype(np.float64), d32.astype(np.float64), p0, p1, p2)
+            hit, t32 = orc.tri_hit(o32, d32, 1e-3, 1e5, f)
+            margin = min(u, v, 1 - u - v)
+            if abs(margin) < 1e-5 or abs(t - 1e-3) < 1e-5:
+                continue
+            expect = margin > 0 and t > 1e-3
+            assert hit == expect, (o, d, f, t, u, v)
+            if hit:
+                assert abs(t32 - t) <= 1e-5 * max(1.0, abs(t))
+                agree += 1
+    assert agree > 500
+
+
+def _fp64_first_round(raw, alpha, u, W, H, k):
+    """All front-face proxy hits in (t_min, t_max) per ray by FP64
+    Moller-Trumbore on the double-precision inverse of the proxy transform,
+    sorted by t; returns per-ray (ids, ambiguous flag)."""
+    packed, act = orc.activate(raw)
+    ids, o2w, w2o = orc.build_proxies(raw, act, alpha)
+    V, I = ico_geometry()
+    p = orc.params_from_uniforms(u, W, H)
+    o = np.array(u.to_world[12:15], np.float64)
+    out = []
+    for py in range(H):
+        for px in range(W):
+            d = orc.ray_dir(p, px, py).astype(np.float64)
+            hits = []
+            amb = False
+            for j in range(len(ids)):
+                A = o2w[j].astype(np.float64).reshape(4, 3).T
+                Ainv = np.linalg.inv(A[:, :3])
+                oo = Ainv @ (o - A[:, 3])
+                od = Ainv @ d
+                for f in range(20):
+                    p0, p1, p2 = V[I[f]]
+                    n = np.cross(p0 - p2, p1 - p0)
+                    if n @ od >= 0:  # back face: n_w . d = n . (W2O d)
+                        if abs(n @ od) < 1e-6:
+                            amb = True
+                        continue
+                    t, uu, vv = mt64(oo, od, p0, p1, p2)
+                    m = min(uu, vv, 1 - uu - vv)
+                    if abs(m) < 1e-5 * (1 + abs(t)):
+                        amb = True
+                    if m < 0 or not (u.t_min < t < u.t_max):
+                        continue
+                    hits.append((t, int(ids[j])))
+            hits.sort()
+            ts = [h[0] for h in hits]
+            if any(abs(a - b) < 1e-5 for a, b in zip(ts, ts[1:])):
+                amb = True
+            out.append(([h[1] for h in hits[:k]], amb))
+    return out
+
+
+def check_kbuffer_bruteforce():
+    raw = gt.generate_raw(n=150, seed=12, config="c1", box_half=0.6, log_scale_mean=-2.0)
+    alpha, W, H, k = 0.05, 12, 12, 4
+    tw = gt.camera_look_at([0.0, 0.0, 3.0], [0.0, 0.0, 0.0])
+    u = gt.make_uniforms(tw, W, H, hfov_deg=40, alpha_min=alpha, k=k, T_min=0.0)
+    ref = _fp64_first_round(raw, alpha, u, W, H, k)
+    s = orc.OracleScene(raw, alpha)
+    r = orc.render(s, orc.params_from_uniforms(u, W, H), max_rounds=8, use_bvh=False)
+    checked = hit_rays = 0
+    for i, (ids, amb) in enumerate(ref):
+        if amb:
+            continue
+        got = [x for x in r["hit_ids"][i, 0] if x != orc.SENTINEL]
+        assert got == ids, (i, got, ids)
+        checked += 1
+        hit_rays += bool(ids)
+    assert checked > 100 and hit_rays > 30
+
+
+def check_response():
+    rng = np.random.default_rng(7)
+    # isotropic: alpha = exp(-b^2 / sigma^2) at impact parameter b (no 1/2)
+    for _ in range(200):
+        mu = rng.normal(size=3)
+        ls = rng.uniform(-3, 0)
+        q = rng.normal(size=4)
+        rec = orc.activate(packed_record(mu, [ls] * 3, q, 2.0))[0][0]
+        sig = float(rec[4])
+        o = (mu + rng.normal(size=3) * 4).astype(F32)
+        d = (mu + rng.normal(size=3) * sig * 0.8 - o)
+        d = (d / np.linalg.norm(d)).astype(F32)
+        m, od, dd = rec[:3].astype(np.float64), o.astype(np.float64), d.astype(np.float64)
+        v = m - od
+        b2 = v @ v - (v @ dd) ** 2 / (dd @ dd)
+        ref = math.exp(-b2 / sig ** 2)
+        got = orc.response(rec, o, d)
+        assert abs(got - ref) <= 3e-5 * max(ref, 1e-3), (got, ref)
+    # ray through the mean: response 1
+    rec = orc.activate(packed_record([0.5, -0.2, 0.1], [-1.0, -2.0, -1.5], [0.3, 0.1, -0.7, 0.2], 1.0))[0][0]
+    d = np.array([0.0, 0.0, -1.0], F32)
+    o = (rec[:3] - 4 * d).astype(F32)
+    assert abs(orc.response(rec, o, d) - 1.0) < 1e-5
+    # anisotropic: exp(-min_t (o + t d - mu)^T Sigma^-1 (o + t d - mu)), Sigma = R S^2 R^T
+    for _ in range(200):
+        mu = rng.normal(size=3)
+        q = rng.normal(size=4)
+        rec = orc.activate(packed_record(mu, rng.uniform(-3, -0.5, 3), q, 2.0))[0][0]
+        R = quat_to_R(rec[8:12].astype(np.float64))
+        Sinv = R @ np.diag(1.0 / rec[4:7].astype(np.float64) ** 2) @ R.T
+        o = (mu + rng.normal(size=3) * 3).astype(F32)
+        d = (mu + rng.normal(size=3) * 0.3 - o)
+        d = (d / np.linalg.norm(d)).astype(F32)
+        od, dd, m = o.astype(np.float64), d.astype(np.float64), rec[:3].astype(np.float64)
+        t = ((m - od) @ Sinv @ dd) / (dd @ Sinv @ dd)
+        e = od + t * dd - m
+        ref = math.exp(-(e @ Sinv @ e))
+        got = orc.response(rec, o, d)
+        assert abs(got - ref) <= 1e-4 * max(ref, 1e-2) + 1e-6, (got, ref)
+
+
+def _real_sh(l, m, d):
+    theta, phi = math.acos(max(-1.0, min(1.0, d[2]))), math.atan2(d[1], d[0])
+    Y = lambda ll, mm: complex(sps.sph_harm_y(ll, mm, theta, phi))
+    if m < 0:
+        return math.sqrt(2) * Y(l, -m).imag
+    if m == 0:
+        return Y(l, 0).real
+    return math.sqrt(2) * Y(l, m).real
+
+
+def check_sh():
+    """Radiance = max(0, 0.5 + sum_lm Y_lm(d) c_lm) with the real SH built from
+    scipy's complex SH (Condon-Shortley phase), sh order (l, m=-l..l)."""
+    rng = np.random.default_rng(8)
+    for _ in range(60):
+        sh = rng.normal(size=(16, 3)) * 0.4
+        rec = np.zeros(80, F32)
+        for i in range(16):
+            rec[16 + 4 * i:16 + 4 * i + 3] = sh[i]
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(F32)
+        dd = d.astype(np.float64)
+        for deg in range(4):
+            basis = [_real_sh(l, m, dd) for l in range(deg + 1) for m in range(-l, l + 1)]
+            ref = np.maximum(0.0, 0.5 + np.array(basis) @ sh[:len(basis)].astype(F32).astype(np.float64))
+            got = orc.radiance(rec, deg, d)
+            assert np.abs(got - ref).max() < 2e-5, (deg, got, ref)
+
+
+def _axis_scene(gaussians, k, T_min=1e-3, alpha=0.01):
+    """Gaussians (mu, log_scale, quat, logit, dc) seen by a 1x1 camera at
+    (0,0,10) looking down -z; returns (oracle result, activated records)."""
+    raw = np.concatenate([packed_record(*g) for g in gaussians])
+    s = orc.OracleScene(raw, alpha)
+    tw = gt.camera_look_at([0.0, 0.0, 10.0], [0.0, 0.0, 0.0])
+    u = gt.make_uniforms(tw, 1, 1, hfov_deg=30, alpha_min=alpha, k=k, T_min=T_min, sh_degree_max=0)
+    r = orc.render(s, orc.params_from_uniforms(u, 1, 1), max_rounds=8, use_bvh=False)
+    return r, s.packed
+
+
+QA = [0.9, 0.3, -0.2, 0.25]   # generic orientations: the axis ray enters through a face interior
+QB = [0.4, -0.5, 0.6, 0.1]
+
+
+def check_composite_closed_form():
+    ga = ([0, 0, 1.0], [-1.5] * 3, QA, 0.4, [0.8, -0.3, 0.1])
+    gb = ([0, 0, -1.0], [-1.2] * 3, QB, 1.1, [-0.2, 0.6, 0.4])
+    r, pk = _axis_scene([ga, gb], k=4)
+    ids = [x for x in r["hit_ids"][0, 0] if x != orc.SENTINEL]
+    assert ids == [0, 1], ids
+    C0 = 0.28209479177387814
+    oa, ob = float(pk[0, 12]), float(pk[1, 12])  # the axis ray passes through both means: response 1
+    ca = np.maximum(0, 0.5 + C0 * pk[0, 16:19].astype(np.float64))
+    cb = np.maximum(0, 0.5 + C0 * pk[1, 16:19].astype(np.float64))
+    ref = oa * ca + (1 - oa) * ob * cb
+    assert np.abs(r["rgba"][0, :3] - ref).max() < 1e-5, (r["rgba"][0], ref)
+    assert abs(r["T"][0] - (1 - oa) * (1 - ob)) < 1e-6
+
+
+def check_relaunch_window():
+    """k=1: a second Gaussian whose proxy is the first one shifted by delta
+    along the ray enters at t1 + delta.  t_curr = t1 + 1e-4 (GS_shader.ts:248)
+    skips it for delta = 5e-5 and keeps it for delta = 3e-4."""
+    C0 = 0.28209479177387814
+    for delta, kept in [(5e-5, False), (3e-4, True)]:
+        ga = ([0, 0, 0.0], [-1.5] * 3, QA, 0.0, [0.8, -0.3, 0.1])
+        gb = ([0, 0, -delta], [-1.5] * 3, QA, 0.0, [-0.2, 0.6, 0.4])
+        r, pk = _axis_scene([ga, gb], k=1)
+        oa = float(pk[0, 12])
+        ca = np.maximum(0, 0.5 + C0 * pk[0, 16:19].astype(np.float64))
+        cb = np.maximum(0, 0.5 + C0 * pk[1, 16:19].astype(np.float64))
+        ref = oa * ca + ((1 - oa) * oa * cb if kept else 0)
+        assert np.abs(r["rgba"][0, :3] - ref).max() < 1e-5, (delta, r["rgba"][0], ref)
+        assert r["n_rounds"][0] == (3 if kept else 2)
+
+
+def check_invariants_and_reeval():
+    raw = gt.generate_raw(n=30000, seed=13, config="c4")
+    s = orc.OracleScene(raw, 0.02)
+    tw = gt.camera_look_at([0.4, -0.3, 3.0], [0, 0, 0])
+    W, H = 40, 32
+    u = gt.make_uniforms(tw, W, H, hfov_deg=45, alpha_min=0.02, k=8, T_min=0.001)
+    p = orc.params_from_uniforms(u, W, H)
+    r = orc.render(s, p, max_rounds=64)
+    assert r["stats"]["overflow"] == 0
+    assert np.all(r["T"] > 0) and np.all(r["T"] <= 1)
+    assert np.all(r["rgba"][:, :3] >= 0) and np.all(r["rgba"][:, 3] == 1)
+    op = s.packed[:, 12]
+    proxy = set(s.proxy_ids.tolist())
+    for i in range(W * H):
+        nr = r["n_rounds"][i]
+        for rd in range(nr):
+            row = r["hit_ids"][i, rd]
+            sent = np.nonzero(row == orc.SENTINEL)[0]
+            if sent.size:
+                assert np.all(row[sent[0]:] == orc.SENTINEL) and rd == nr - 1
+            for g in row[row != orc.SENTINEL]:
+                assert int(g) in proxy and op[g] > 0.0
+            acc = r["accept"][i, rd]
+            assert acc >> int(sent[0] if sent.size else u.k) == 0
+        assert np.all(r["hit_ids"][i, nr:] == orc.SENTINEL)
+    assert r["stats"]["h_comp"] > 500 and (r["n_rounds"] > 1).sum() > 50
+    rgb64, T64 = orc.reeval_f64(s, p, r["n_rounds"], r["hit_ids"], r["accept"])
+    assert np.abs(r["rgba"][:, :3] - rgb64).max() < 1e-4
+    assert np.abs(r["T"] - T64).max() < 1e-5
+
+
+PINS = {
+    "expf": check_expf, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
+    "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
+    "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
+    "composite": check_composite_closed_form, "relaunch": check_relaunch_window,
+    "invariants": check_invariants_and_reeval,
+}
+
+
+@pytest.mark.parametrize("name", list(PINS))
+def test_pin(name):
+    orc.set_mutation(0)
+    PINS[name]()
+
+
+# mutation -> the pin that must catch it
+MUTATIONS = [
+    (1, "response"),          # 1/2 factor in the exponent (GS_shader.ts:135 has none)
+    (2, "sh"),                # sign of the degree-1 y term flipped
+    (3, "ray_dir"),           # camera X axis not negated (GS_shader.ts:277)
+    (4, "kbuffer_bruteforce"),  # facing test inverted (trace.glsl:159)
+    (5, "relaunch"),          # relaunch without the 1e-4 step (GS_shader.ts:248)
+    (6, "composite"),         # transmittance dropped from L += T*alpha*c
+    (7, "proxy_tangent"),     # proxy scale s = sqrt(2 ln(o/alpha)) dropped (main.ts:84)
+]
+
+
+@pytest.mark.parametrize("mutation,pin", MUTATIONS)
+def test_mutation_detected(mutation, pin):
+    orc.set_mutation(mutation)
+    try:
+        with pytest.raises(AssertionError):
+            PINS[pin]()
+    finally:
+        orc.set_mutation(0)

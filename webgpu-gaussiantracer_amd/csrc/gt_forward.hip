@@ -1,0 +1,461 @@
+/*
+ * gt_forward.hip - fused forward kernel of the Gaussian ray tracer (gfx950).
+ *
+ * One lane = one ray (pixel).  Per ray, rounds of
+ *   traverse the 4-wide BVH over icosahedron proxies, keeping the k
+ *   closest front-face hits in a register k-buffer          (any-hit, GS_shader.ts:323-368)
+ *   t_curr = t_k + 1e-4                                      (GS_shader.ts:248)
+ *   composite the k entries front to back with max-response
+ *   opacity and SH radiance                                  (GS_shader.ts:250-269, 111-190)
+ * until T <= T_min or the ray leaves the scene (GS_shader.ts:231).  This
+ * replaces the ray-gen -> traceRayEXT -> any-hit chain of the reference's
+ * emulated RT pipeline (webrtx/src/glsl/trace.glsl:42-245) by one kernel.
+ *
+ * Bit-exactness with a host C evaluation: the ray, the object-space ray, the
+ * triangle test, the facing test, the response and exp are the shared
+ * primitives of gt_fp32.h, compiled with -ffp-contract=off, IEEE division
+ * and square root and f32 denormals kept.  Every shortcut that skips work
+ * (box culling, the bounding-sphere test, the facing pre-test) is
+ * conservative with margins far above FP32 rounding, so it never changes
+ * which hits are found.  Exact-t ties are ordered by (t, Gaussian id).
+ *
+ * Launch: 256-thread blocks = 4 waves; a wave covers an 8x8 pixel tile;
+ * block ids are remapped so that consecutive tiles share an XCD (and its L2).
+ */
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "../../include/gt.h"
+#include "../../include/gt_fp32.h"
+#include "gt_internal.h"
+
+namespace {
+
+/* Face constants {p0, e0, e1, n} of the 20 icosahedron faces, uploaded once
+ * per device from gt_ico_face (scalar loads in the face loop). */
+__constant__ float c_faces[12 * GT_NUM_FACES];
+
+/* max over faces of |n|_1 is 2.63; 3 bounds it */
+#define GT_FACE_N1_BOUND 3.0f
+
+struct RayState {
+  gt_f3 o, d;
+  gt_f3 inv; /* slab-test reciprocals, tiny components clamped */
+};
+
+__device__ __forceinline__ bool key_lt(float ta, uint32_t ia, float tb, uint32_t ib) {
+  const uint32_t a = __float_as_uint(ta), b = __float_as_uint(tb);
+  return a < b || (a == b && ia < ib);
+}
+
+/* k-buffer in registers.  The k active slots are the LAST k of MAXK; the
+ * first MAXK-k hold the key (+0, 0), smaller than every hit key (hits have
+ * t > t_curr >= 0), so the insertion chain never moves them and the k-th
+ * entry is always slot MAXK-1. */
+template <int MAXK>
+struct KBuf {
+  float t[MAXK];
+  uint32_t id[MAXK];
+
+  __device__ __forceinline__ void reset(uint32_t off, float t_max) {
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) {
+      const bool dummy = (uint32_t)i < off;
+      t[i] = dummy ? 0.0f : t_max;
+      id[i] = dummy ? 0u : GT_SENTINEL;
+    }
+  }
+  /* GS_shader.ts:349-353 with the key (t, id): stable for equal keys */
+  __device__ __forceinline__ void insert(float ht, uint32_t hid) {
+    if (!key_lt(ht, hid, t[MAXK - 1], id[MAXK - 1])) return;
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) {
+      const bool sw = key_lt(ht, hid, t[i], id[i]);
+      const float tt = t[i];
+      const uint32_t ti = id[i];
+      t[i] = sw ? ht : tt;
+      id[i] = sw ? hid : ti;
+      ht = sw ? tt : ht;
+      hid = sw ? ti : hid;
+    }
+  }
+  __device__ __forceinline__ void shift_left(int s) {
+#pragma unroll
+    for (int i = 0; i + 1 < MAXK; ++i) {
+      if (i + s < MAXK) {
+        t[i] = t[i + s];
+        id[i] = id[i + s];
+      }
+    }
+  }
+};
+
+struct Counters {
+  uint32_t rounds = 0, h_eval = 0, h_comp = 0, proxies = 0, hits = 0, nodes = 0, overflow = 0;
+};
+
+enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
+
+/* Exact proxy test: object-space ray (trace.glsl:102-105), 20 faces with the
+ * triangle test of intersect.glsl:73-92 and the facing rule of
+ * trace.glsl:153-164.  MODE_CLOSEST keeps back faces (debugMode 1 traces with
+ * no cull flag, GS_shader.ts:292-294). */
+template <int MAXK, int MODE>
+__device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, const RayState& r, float tmin,
+                                           float tmax, KBuf<MAXK>& kb, uint32_t& count, Counters& st) {
+  const float4* P = reinterpret_cast<const float4*>(S.prox) + 4 * (size_t)pi;
+  const float4 q0 = P[0], q1 = P[1], q2 = P[2], q3 = P[3];
+  const float w[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+  const uint32_t gid = __float_as_uint(q3.x);
+  st.proxies++;
+  const gt_f3 oo = gt_xform_point(w, r.o);
+  const gt_f3 od = gt_xform_dir(w, r.d);
+  /* conservative bounding-sphere test: every face lies inside the
+   * circumsphere; the margin covers the FP32 error of the cross product */
+  {
+    const gt_f3 cr = gt_cross(oo, od);
+    const float o1 = fabsf(oo.x) + fabsf(oo.y) + fabsf(oo.z);
+    const float R = GT_ICO_CIRCUMRADIUS * 1.001f + 1e-4f * o1;
+    if (gt_dot(cr, cr) > R * R * gt_dot(od, od)) return;
+  }
+  float margin = 0.0f;
+  if (MODE != MODE_CLOSEST) {
+    /* facing pre-test: dot(n, d_obj) and the exact facing dot(n*W2O, d)
+     * differ by < 8 ulp * |n|_1 * sum|W2O|; faces with dot(n, d_obj) above
+     * this bound are certainly culled as back faces */
+    float ws = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ws += fabsf(w[i]);
+    margin = 1e-5f * GT_FACE_N1_BOUND * ws;
+  }
+#pragma unroll 2
+  for (int f = 0; f < GT_NUM_FACES; ++f) {
+    gt_face fc;
+    fc.p0 = gt_f3_make(c_faces[12 * f + 0], c_faces[12 * f + 1], c_faces[12 * f + 2]);
+    fc.e0 = gt_f3_make(c_faces[12 * f + 3], c_faces[12 * f + 4], c_faces[12 * f + 5]);
+    fc.e1 = gt_f3_make(c_faces[12 * f + 6], c_faces[12 * f + 7], c_faces[12 * f + 8]);
+    fc.n = gt_f3_make(c_faces[12 * f + 9], c_faces[12 * f + 10], c_faces[12 * f + 11]);
+    if (MODE != MODE_CLOSEST) {
+      if (gt_dot(fc.n, od) > margin) continue;
+    }
+    float t;
+    if (!gt_tri_hit(oo, od, tmin, tmax, fc, &t)) continue;
+    if (MODE != MODE_CLOSEST && !gt_front_facing(w, fc.n, r.d)) continue;
+    st.hits++;
+    if (MODE == MODE_COUNT) {
+      count++;
+    } else {
+      kb.insert(t, gid);
+    }
+  }
+}
+
+/* One traversal of the BVH for the interval (tmin, tmax): collects the
+ * k smallest (t, id) hits into kb.  Children are visited near to far and
+ * culled against the current k-th entry (with conservative slack). */
+template <int MAXK, int MODE>
+__device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, float tmax, float slack_abs,
+                      KBuf<MAXK>& kb, uint32_t& count, Counters& st) {
+  if (S.root == GT_NODE_EMPTY) return;
+  uint32_t stack[GT_STACK_MAX];
+  int sp = 0;
+  stack[sp++] = S.root;
+  const float lo_t = tmin * (1.0f - 2e-5f) - slack_abs;
+  while (sp > 0) {
+    const uint32_t ref = stack[--sp];
+    if (ref & GT_LEAF_BIT) {
+      const uint32_t first = ref & GT_LEAF_FIRST_MASK;
+      const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+      for (uint32_t j = 0; j < cnt; ++j) test_proxy<MAXK, MODE>(S, first + j, r, tmin, tmax, kb, count, st);
+      continue;
+    }
+    st.nodes++;
+    const float4* N = reinterpret_cast<const float4*>(S.nodes + ref);
+    const float4 lx = N[0], hx = N[1], ly = N[2], hy = N[3], lz = N[4], hz = N[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(N)[6];
+    const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
+    const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
+    const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+    float tn[4];
+    uint32_t rr[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float ax = (clx[c] - r.o.x) * r.inv.x, bx = (chx[c] - r.o.x) * r.inv.x;
+      const float ay = (cly[c] - r.o.y) * r.inv.y, by = (chy[c] - r.o.y) * r.inv.y;
+      const float az = (clz[c] - r.o.z) * r.inv.z, bz = (chz[c] - r.o.z) * r.inv.z;
+      const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+      const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+      const bool hit = tnear <= tfar && cref[c] != GT_NODE_EMPTY;
+      tn[c] = hit ? tnear : INFINITY;
+      rr[c] = hit ? cref[c] : GT_NODE_EMPTY;
+    }
+    /* sort descending by tnear (misses = +inf first), push far to near */
+#define GT_CSWAP(a, b)                                   \
+  if (tn[a] < tn[b]) {                                   \
+    const float t_ = tn[a]; tn[a] = tn[b]; tn[b] = t_;   \
+    const uint32_t r_ = rr[a]; rr[a] = rr[b]; rr[b] = r_; \
+  }
+    GT_CSWAP(0, 1) GT_CSWAP(2, 3) GT_CSWAP(0, 2) GT_CSWAP(1, 3) GT_CSWAP(1, 2)
+#undef GT_CSWAP
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (rr[c] != GT_NODE_EMPTY) stack[sp++] = rr[c];
+  }
+}
+
+/* shToRadiance + computeRadiance (GS_shader.ts:138-190) */
+__device__ __forceinline__ gt_f3 radiance(const float* __restrict__ shp, uint32_t deg, gt_f3 dir) {
+  const float4* S4 = reinterpret_cast<const float4*>(shp);
+  float sh[48];
+  const int n4 = deg == 0 ? 1 : deg == 1 ? 3 : deg == 2 ? 7 : 12;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    if (i < n4) {
+      const float4 v = S4[i];
+      sh[4 * i] = v.x; sh[4 * i + 1] = v.y; sh[4 * i + 2] = v.z; sh[4 * i + 3] = v.w;
+    } else {
+      sh[4 * i] = sh[4 * i + 1] = sh[4 * i + 2] = sh[4 * i + 3] = 0.0f;
+    }
+  }
+  const float x = dir.x, y = dir.y, z = dir.z;
+  const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  float out[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#define SH(i) sh[3 * (i) + c]
+    float L = 0.5f + 0.28209479177387814f * SH(0);
+    if (deg >= 1) L += 0.4886025119029199f * ((-y * SH(1) + z * SH(2)) - x * SH(3));
+    if (deg >= 2)
+      L += (((1.0925484305920792f * xy * SH(4) + -1.0925484305920792f * yz * SH(5)) +
+             0.31539156525252005f * ((2.f * zz - xx) - yy) * SH(6)) +
+            -1.0925484305920792f * xz * SH(7)) +
+           0.5462742152960396f * (xx - yy) * SH(8);
+    if (deg >= 3)
+      L += (((((-0.5900435899266435f * y * (3.0f * xx - yy) * SH(9) + 2.890611442640554f * xy * z * SH(10)) +
+               -0.4570457994644658f * y * ((4.0f * zz - xx) - yy) * SH(11)) +
+              0.3731763325901154f * z * ((2.0f * zz - 3.0f * xx) - 3.0f * yy) * SH(12)) +
+             -0.4570457994644658f * x * ((4.0f * zz - xx) - yy) * SH(13)) +
+            1.445305721320277f * z * (xx - yy) * SH(14)) +
+           -0.5900435899266435f * x * (xx - 3.0f * yy) * SH(15);
+#undef SH
+    out[c] = L > 0.0f ? L : 0.0f;
+  }
+  return gt_f3_make(out[0], out[1], out[2]);
+}
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, slot = b >> 3;
+  const uint32_t q = nb >> 3, rem = nb & 7u;
+  return x * q + (x < rem ? x : rem) + slot;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int MAXK, int MODE, bool DUMP, bool STATS>
+__global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_dev_frame F) {
+  const uint32_t tiles_x = (F.width + 7u) >> 3;
+  const uint32_t tiles_y = (F.height + 7u) >> 3;
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t tile = blk * 4u + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+  const bool active = tile < tiles_x * tiles_y && px < F.width && py < F.height;
+  Counters st;
+  if (active) {
+    const uint32_t ray = py * F.width + px;
+    RayState r;
+    r.o = gt_f3_make(F.to_world[12], F.to_world[13], F.to_world[14]);
+    r.d = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
+    {
+      const float dx = fabsf(r.d.x) < 1e-20f ? copysignf(1e-20f, r.d.x) : r.d.x;
+      const float dy = fabsf(r.d.y) < 1e-20f ? copysignf(1e-20f, r.d.y) : r.d.y;
+      const float dz = fabsf(r.d.z) < 1e-20f ? copysignf(1e-20f, r.d.z) : r.d.z;
+      r.inv = gt_f3_make(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    }
+    gt_f3 L = gt_f3_make(0.0f, 0.0f, 0.0f);
+    float T = 1.0f;
+    uint32_t rounds = 0;
+    const size_t MR = F.max_rounds;
+    KBuf<MAXK> kb;
+    uint32_t count = 0;
+    if (MODE == MODE_RENDER) {
+      const uint32_t k = F.k;
+      const uint32_t off = (uint32_t)MAXK - k;
+      float t_curr = F.t_min;
+      uint32_t step = 0;
+      while (F.T_min < T && t_curr < F.t_max) {
+        if (F.early_stop == step++) break;
+        kb.reset(off, F.t_max);
+        trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st);
+        t_curr = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
+        /* bring the active slots to the front (uniform shifts) */
+        if (off & 1u) kb.shift_left(1);
+        if (off & 2u) kb.shift_left(2);
+        if (off & 4u) kb.shift_left(4);
+        if (off & 8u) kb.shift_left(8);
+        if (off & 16u) kb.shift_left(16);
+        bool stop = false;
+        uint32_t mask = 0;
+        for (uint32_t j = 0; j < k; ++j) {
+          const uint32_t id = kb.id[0];
+          if (DUMP && rounds < MR) F.hit_ids[((size_t)ray * MR + rounds) * k + j] = id;
+          if (!stop) {
+            if (id == GT_SENTINEL) {
+              stop = true;
+              t_curr = F.t_max;
+            } else {
+              st.h_eval++;
+              const float4* G = reinterpret_cast<const float4*>(S.geom) + 3 * (size_t)id;
+              const float4 g0 = G[0], g1 = G[1], g2 = G[2];
+              const float a6[6] = {g1.x, g1.y, g1.z, g1.w, g2.x, g2.y};
+              const gt_f3 mu = gt_f3_make(g0.x, g0.y, g0.z);
+              const float alpha = gt_response(a6, mu, r.o, r.d) * g0.w;
+              if (F.alpha_min < alpha) {
+                const gt_f3 dir = gt_normalize(gt_sub(mu, r.o));
+                const gt_f3 c = radiance(S.sh + 48 * (size_t)id, F.sh_degree, dir);
+                const float Ta = T * alpha;
+                L = gt_add(L, gt_scale(Ta, c));
+                T *= 1.0f - alpha;
+                mask |= 1u << j;
+                st.h_comp++;
+              }
+            }
+          }
+          kb.shift_left(1);
+        }
+        if (DUMP && rounds < MR) F.accept[(size_t)ray * MR + rounds] = mask;
+        if (rounds >= MR) st.overflow++;
+        rounds++;
+      }
+    } else if (MODE == MODE_CLOSEST) {
+      /* debugMode 1 (GS_shader.ts:289-296, 312-321) */
+      kb.reset(0, F.t_max);
+      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st);
+      const uint32_t id = kb.id[MAXK - 1];
+      if (id != GT_SENTINEL) L = radiance(S.sh + 48 * (size_t)id, F.sh_degree, r.d);
+      rounds = 1;
+      if (DUMP && MR > 0) {
+        F.hit_ids[(size_t)ray * MR * F.k] = id;
+        F.accept[(size_t)ray * MR] = 0;
+      }
+    } else {
+      /* debugMode 2 (GS_shader.ts:201-228): front-face hits in (t_min, t_max) */
+      kb.reset(0, F.t_max);
+      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st);
+      const uint32_t c = count;
+      float cr, cg, cb;
+      if (c == 0) { cr = 0.f; cg = 0.f; cb = 0.f; }
+      else if (c < 20) { cr = 0.5f; cg = 0.5f; cb = 0.f; }
+      else if (c < 40) { cr = 0.5f; cg = 0.f; cb = 0.5f; }
+      else if (c < 60) { cr = 0.f; cg = 0.5f; cb = 0.5f; }
+      else if (c < 80) { cr = 0.5f; cg = 0.f; cb = 0.f; }
+      else if (c < 100) { cr = 0.f; cg = 0.5f; cb = 0.f; }
+      else if (c < 150) { cr = 0.f; cg = 0.f; cb = 0.5f; }
+      else { cr = 0.5f; cg = 0.5f; cb = 0.5f; }
+      L = gt_f3_make(cr, cg, cb);
+      rounds = 1;
+      if (DUMP && MR > 0) {
+        F.hit_ids[(size_t)ray * MR * F.k] = c;
+        F.accept[(size_t)ray * MR] = 0;
+      }
+    }
+    reinterpret_cast<float4*>(F.rgba)[ray] = make_float4(L.x, L.y, L.z, 1.0f);
+    if (F.n_rounds) F.n_rounds[ray] = rounds;
+    if (F.T_out) F.T_out[ray] = T;
+    if (DUMP) {
+      const uint32_t kk = MODE == MODE_RENDER ? F.k : F.k;
+      const uint32_t first_unwritten = MODE == MODE_RENDER ? rounds : 1u;
+      for (size_t rd = first_unwritten; rd < MR; ++rd) {
+        for (uint32_t j = 0; j < kk; ++j) F.hit_ids[((size_t)ray * MR + rd) * kk + j] = GT_SENTINEL;
+        F.accept[(size_t)ray * MR + rd] = 0;
+      }
+      if (MODE != MODE_RENDER && MR > 0)
+        for (uint32_t j = 1; j < kk; ++j) F.hit_ids[(size_t)ray * MR * kk + j] = GT_SENTINEL;
+    }
+    st.rounds = rounds;
+  }
+  if (STATS) {
+    const uint32_t v[8] = {active ? 1u : 0u, st.rounds, st.h_eval, st.h_comp, st.proxies, st.hits, st.nodes, st.overflow};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t s = wave_sum(v[i]);
+      if (lane == 0 && s) atomicAdd(F.stats + i, (unsigned long long)s);
+    }
+  }
+}
+
+template <int MAXK, int MODE>
+hipError_t launch4(const gt_dev_scene& S, const gt_dev_frame& F, hipStream_t st, bool dump, bool stats) {
+  const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
+  const dim3 grid((tiles + 3u) / 4u), block(256);
+  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true>), grid, block, 0, st, S, F);
+  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false>), grid, block, 0, st, S, F);
+  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true>), grid, block, 0, st, S, F);
+  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false>), grid, block, 0, st, S, F);
+  return hipGetLastError();
+}
+
+std::mutex g_init_mu;
+bool g_faces_ready[64];
+
+hipError_t ensure_faces() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (dev >= 0 && dev < 64 && g_faces_ready[dev]) return hipSuccess;
+  float f[12 * GT_NUM_FACES];
+  for (int i = 0; i < GT_NUM_FACES; ++i) {
+    const gt_face fc = gt_ico_face(i);
+    const float v[12] = {fc.p0.x, fc.p0.y, fc.p0.z, fc.e0.x, fc.e0.y, fc.e0.z,
+                         fc.e1.x, fc.e1.y, fc.e1.z, fc.n.x,  fc.n.y,  fc.n.z};
+    for (int j = 0; j < 12; ++j) f[12 * i + j] = v[j];
+  }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_faces), f, sizeof(f), 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess && dev >= 0 && dev < 64) g_faces_ready[dev] = true;
+  return e;
+}
+
+}  // namespace
+
+extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, void* stream) {
+  hipError_t e = ensure_faces();
+  if (e != hipSuccess) {
+    gt_set_error("face table upload failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  if ((F->hit_ids == nullptr) != (F->accept == nullptr)) {
+    gt_set_error("hit_ids and accept must be requested together");
+    return GT_E_INVALID;
+  }
+  const bool dump = F->hit_ids != nullptr && F->max_rounds > 0;
+  const bool stats = F->stats != nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  if (F->debug_mode == 1) {
+    e = launch4<1, MODE_CLOSEST>(*S, *F, st, dump, stats);
+  } else if (F->debug_mode == 2) {
+    e = launch4<1, MODE_COUNT>(*S, *F, st, dump, stats);
+  } else if (F->k <= 4) {
+    e = launch4<4, MODE_RENDER>(*S, *F, st, dump, stats);
+  } else if (F->k <= 8) {
+    e = launch4<8, MODE_RENDER>(*S, *F, st, dump, stats);
+  } else if (F->k <= 16) {
+    e = launch4<16, MODE_RENDER>(*S, *F, st, dump, stats);
+  } else {
+    e = launch4<32, MODE_RENDER>(*S, *F, st, dump, stats);
+  }
+  if (e != hipSuccess) {
+    gt_set_error("forward launch failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  return GT_OK;
+}

@@ -1,0 +1,79 @@
+/*
+ * gt_internal.h - layouts shared by the host builder (gt_scene.cpp) and the
+ * HIP kernels (gt_forward.hip).
+ *
+ * HBM layout of a scene (all arrays 256-B aligned):
+ *   geom   N  x 48 B  {mu.xyz, opacity} {a00 a01 a02 a11} {a12 a22 0 0}
+ *                     a = inverse covariance of computeResponse
+ *                     (src/GS_shader.ts:115-130), precomputed with the
+ *                     same FP32 operations (gt_inv_cov), so the response
+ *                     is bit-identical to evaluating it per hit.
+ *   sh     N  x 192 B sh[16][3] without the vec3 padding of the 320-B record
+ *   prox   N' x 64 B  {w2o c0.xyz c1.x} {c1.yz c2.xy} {c2.z c3.xyz} {gid 0 0 0}
+ *                     stored in BVH leaf order, so a leaf is a contiguous run
+ *   nodes  M  x 128 B 4-wide BVH node: child boxes as SoA float4 rows
+ *   faces  20 x 48 B  icosahedron face constants {p0, e0, e1, n}
+ */
+#ifndef GT_INTERNAL_H
+#define GT_INTERNAL_H
+
+#include <stdint.h>
+
+#define GT_NODE_EMPTY 0xffffffffu
+#define GT_LEAF_BIT 0x80000000u
+#define GT_LEAF_MAX 8u                /* proxies per leaf */
+#define GT_LEAF_COUNT_SHIFT 28        /* leaf ref: 1 | (count-1):3 | first:28 */
+#define GT_LEAF_FIRST_MASK 0x0fffffffu
+#define GT_MAX_PROXIES GT_LEAF_FIRST_MASK
+#define GT_STACK_MAX 96               /* traversal stack entries per ray */
+
+static inline uint32_t gt_leaf_ref(uint32_t first, uint32_t count) {
+  return GT_LEAF_BIT | ((count - 1u) << GT_LEAF_COUNT_SHIFT) | first;
+}
+
+/* 128-B BVH4 node.  Row r of lo/hi holds axis r of the four child boxes. */
+typedef struct gt_node4 {
+  float lo_x[4], hi_x[4];
+  float lo_y[4], hi_y[4];
+  float lo_z[4], hi_z[4];
+  uint32_t child[4]; /* inner node index, leaf ref, or GT_NODE_EMPTY */
+  uint32_t _pad[4];
+} gt_node4;
+
+typedef struct gt_dev_scene {
+  const float* geom;     /* N x 12 */
+  const float* sh;       /* N x 48 */
+  const float* prox;     /* N' x 16 */
+  const gt_node4* nodes; /* M */
+  const float* faces;    /* 20 x 12 */
+  uint32_t root;         /* ref of the root (inner index, leaf ref or EMPTY) */
+  uint32_t n, n_prox, n_nodes;
+} gt_dev_scene;
+
+typedef struct gt_dev_frame {
+  float to_world[16];
+  float ax, ay;
+  float t_min, t_max, T_min, alpha_min;
+  uint32_t k, sh_degree, early_stop, debug_mode;
+  uint32_t width, height;
+  float slack_abs; /* box-test slack, |camera origin| dependent */
+  uint32_t max_rounds;
+  float* rgba;
+  uint32_t* hit_ids;
+  uint32_t* accept;
+  uint32_t* n_rounds;
+  float* T_out;
+  unsigned long long* stats;
+} gt_dev_frame;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* implemented in gt_forward.hip */
+int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, void* stream);
+void gt_set_error(const char* fmt, ...);
+#ifdef __cplusplus
+}
+#endif
+
+#endif

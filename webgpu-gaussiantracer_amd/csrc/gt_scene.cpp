@@ -1,0 +1,742 @@
+/*
+ * gt_scene.cpp - scene (acceleration container) construction and the
+ * gt_render host wrapper.
+ *
+ * Replaces, for the Gaussian path:
+ *   PackedGaussians activations + packing   src/GS_ply.ts:187-252
+ *   gaussianSceneToBvh (filter, proxies)    src/main.ts:30-111
+ *   TLAS/BLAS build + upload                webrtx/src/wasm_bvh_builder.ts:260-377,
+ *                                           webrtx/bvh/src/lib.rs:253-347, 482-603
+ * The reference builds a binary SAH TLAS over instance boxes and walks a
+ * 39-node BLAS per instance; here a binned-SAH binary tree over padded
+ * world boxes of the proxies is collapsed into 4-wide 128-B nodes whose
+ * leaves hold runs of 64-B proxy records (world-to-object matrix + id), and
+ * the 20 icosahedron faces are tested directly (no BLAS).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/gt.h"
+#include "../../include/gt_fp32.h"
+#include "gt_internal.h"
+
+namespace {
+
+struct HostScene {
+  uint32_t n = 0;
+  std::vector<float> raw;      // n x 59 (empty for packed input)
+  std::vector<double> act;     // n x 8: opacity, scale[3], quat[4] (JS doubles)
+  std::vector<float> packed;   // n x 80
+  // proxies, in original order
+  std::vector<uint32_t> proxy_ids;
+  std::vector<float> o2w, w2o;  // n' x 12
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct gt_scene {
+  HostScene h;
+  double alpha_min = 0.0;
+  // device
+  int device = 0;
+  DevBuf geom, sh, prox, nodes, faces;
+  gt_dev_scene dev{};
+  uint32_t n_nodes = 0, max_stack = 0, depth = 0;
+  double build_ms = 0.0;
+};
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      gt_set_error("%s failed: %s", #expr, hipGetErrorString(e_));                 \
+      return GT_E_HIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+/* ---------------- activations (src/GS_ply.ts:187-196, 225-235) ---------------- */
+static void activate_raw(HostScene& h) {
+  h.act.assign((size_t)h.n * 8, 0.0);
+  h.packed.assign((size_t)h.n * 80, 0.0f);
+  for (uint32_t i = 0; i < h.n; ++i) {
+    const float* r = &h.raw[(size_t)i * GT_RAW_FLOATS];
+    float* o = &h.packed[(size_t)i * 80];
+    double* a = &h.act[(size_t)i * 8];
+    double sc0 = std::exp((double)r[52]), sc1 = std::exp((double)r[53]), sc2 = std::exp((double)r[54]);
+    double q0 = r[55], q1 = r[56], q2 = r[57], q3 = r[58];
+    double norm = std::sqrt(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
+    q0 /= norm; q1 /= norm; q2 /= norm; q3 /= norm;
+    double op = 1.0 / (1.0 + std::exp(-(double)r[51]));
+    o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
+    o[4] = (float)sc0; o[5] = (float)sc1; o[6] = (float)sc2;
+    o[8] = (float)q0; o[9] = (float)q1; o[10] = (float)q2; o[11] = (float)q3;
+    o[12] = (float)op;
+    for (int c = 0; c < 3; ++c) o[16 + c] = r[3 + c];
+    for (int j = 0; j < 15; ++j)
+      for (int c = 0; c < 3; ++c) o[16 + 4 * (1 + j) + c] = r[6 + c * 15 + j];
+    a[0] = op; a[1] = sc0; a[2] = sc1; a[3] = sc2;
+    a[4] = q0; a[5] = q1; a[6] = q2; a[7] = q3;
+  }
+}
+
+static void activate_packed(HostScene& h) {
+  h.act.assign((size_t)h.n * 8, 0.0);
+  for (uint32_t i = 0; i < h.n; ++i) {
+    const float* o = &h.packed[(size_t)i * 80];
+    double* a = &h.act[(size_t)i * 8];
+    a[0] = o[12];
+    for (int j = 0; j < 3; ++j) a[1 + j] = o[4 + j];
+    for (int j = 0; j < 4; ++j) a[4 + j] = o[8 + j];
+  }
+}
+
+/* ---------------- proxies (src/main.ts:73-94, webrtx/bvh/src/lib.rs:501-505) ---------------- */
+/* Rotation of wgpu-matrix mat3.fromQuat([x, y, z, w]) evaluated in double
+ * and stored as f32; R[c][r] = column c, row r. */
+static void quat_to_mat_f32(double x, double y, double z, double w, float R[3][3]) {
+  const double x2 = x + x, y2 = y + y, z2 = z + z;
+  const double xx = x * x2, yx = y * x2, yy = y * y2;
+  const double zx = z * x2, zy = z * y2, zz = z * z2;
+  const double wx = w * x2, wy = w * y2, wz = w * z2;
+  R[0][0] = (float)(1 - yy - zz); R[0][1] = (float)(yx + wz); R[0][2] = (float)(zx - wy);
+  R[1][0] = (float)(yx - wz); R[1][1] = (float)(1 - xx - zz); R[1][2] = (float)(zy + wx);
+  R[2][0] = (float)(zx + wy); R[2][1] = (float)(zy - wx); R[2][2] = (float)(1 - xx - yy);
+}
+
+/* glam Affine3A::inverse (f32, scalar build): rows of the inverse 3x3 are
+ * (y x z, z x x, x x y) / det(z . (x x y)), translation = -(inv * t). */
+static void invert_affine_f32(const float* m, float* out) {
+  const gt_f3 cx = gt_f3_make(m[0], m[1], m[2]);
+  const gt_f3 cy = gt_f3_make(m[3], m[4], m[5]);
+  const gt_f3 cz = gt_f3_make(m[6], m[7], m[8]);
+  const gt_f3 row0 = gt_cross(cy, cz), row1 = gt_cross(cz, cx), row2 = gt_cross(cx, cy);
+  const float inv_det = 1.0f / gt_dot(cz, row2);
+  const gt_f3 a = gt_scale(inv_det, row0), b = gt_scale(inv_det, row1), c = gt_scale(inv_det, row2);
+  const gt_f3 k0 = gt_f3_make(a.x, b.x, c.x), k1 = gt_f3_make(a.y, b.y, c.y), k2 = gt_f3_make(a.z, b.z, c.z);
+  const gt_f3 it = gt_add(gt_add(gt_scale(m[9], k0), gt_scale(m[10], k1)), gt_scale(m[11], k2));
+  const float v[12] = {k0.x, k0.y, k0.z, k1.x, k1.y, k1.z, k2.x, k2.y, k2.z, -it.x, -it.y, -it.z};
+  std::memcpy(out, v, sizeof(v));
+}
+
+static void build_proxies(HostScene& h, double alpha_min) {
+  h.proxy_ids.clear();
+  h.o2w.clear();
+  h.w2o.clear();
+  for (uint32_t i = 0; i < h.n; ++i) {
+    const double* a = &h.act[(size_t)i * 8];
+    const double opacity = a[0];
+    if (!(opacity > alpha_min)) continue;
+    const double s = std::sqrt(2 * std::log(opacity / alpha_min));
+    float S[3];
+    for (int j = 0; j < 3; ++j) S[j] = (float)(s * a[1 + j]);
+    float R[3][3];
+    quat_to_mat_f32(a[5], a[6], a[7], a[4], R);
+    float m[12];
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) m[c * 3 + r] = (float)((double)R[c][r] * (double)S[c]);
+    const float* pos = &h.packed[(size_t)i * 80];
+    m[9] = pos[0]; m[10] = pos[1]; m[11] = pos[2];
+    float inv[12];
+    invert_affine_f32(m, inv);
+    h.proxy_ids.push_back(i);
+    h.o2w.insert(h.o2w.end(), m, m + 12);
+    h.w2o.insert(h.w2o.end(), inv, inv + 12);
+  }
+}
+
+/* ---------------- BVH ---------------- */
+namespace {
+
+struct Box {
+  float lo[3], hi[3];
+  void empty() {
+    for (int j = 0; j < 3; ++j) { lo[j] = INFINITY; hi[j] = -INFINITY; }
+  }
+  void grow(const Box& b) {
+    for (int j = 0; j < 3; ++j) { lo[j] = std::min(lo[j], b.lo[j]); hi[j] = std::max(hi[j], b.hi[j]); }
+  }
+  double area() const {
+    double dx = std::max(0.0f, hi[0] - lo[0]), dy = std::max(0.0f, hi[1] - lo[1]), dz = std::max(0.0f, hi[2] - lo[2]);
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+  }
+};
+
+struct B2Node {
+  Box box;
+  int32_t left = -1, right = -1;  // leaf if left < 0
+  uint32_t first = 0, count = 0;
+};
+
+/* World box of a proxy: the 12 icosahedron vertices through O2W in double,
+ * padded by far more than the FP32 error of the object-space triangle test,
+ * then rounded outward to f32.  Culling with these boxes never drops a hit. */
+static Box proxy_box(const float* m) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int v = 0; v < 12; ++v) {
+    const gt_f3 p = gt_ico_vertex(v);
+    for (int r = 0; r < 3; ++r) {
+      const double w = (double)m[r] * p.x + (double)m[3 + r] * p.y + (double)m[6 + r] * p.z + (double)m[9 + r];
+      lo[r] = std::min(lo[r], w);
+      hi[r] = std::max(hi[r], w);
+    }
+  }
+  Box b;
+  for (int r = 0; r < 3; ++r) {
+    const double pad = 1e-3 * (hi[r] - lo[r]) + 2e-5 * (1.0 + std::fabs(lo[r]) + std::fabs(hi[r]));
+    float l = (float)(lo[r] - pad), u = (float)(hi[r] + pad);
+    if ((double)l > lo[r] - pad) l = std::nextafter(l, -INFINITY);
+    if ((double)u < hi[r] + pad) u = std::nextafter(u, INFINITY);
+    b.lo[r] = l;
+    b.hi[r] = u;
+  }
+  return b;
+}
+
+struct Builder {
+  const std::vector<Box>& boxes;
+  std::vector<float> cent;  // 3 per prim
+  std::vector<uint32_t> order;
+  std::vector<B2Node> nodes;
+  uint32_t max_depth = 0;
+
+  explicit Builder(const std::vector<Box>& b) : boxes(b) {
+    const size_t n = b.size();
+    cent.resize(3 * n);
+    order.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      order[i] = (uint32_t)i;
+      for (int j = 0; j < 3; ++j) cent[3 * i + j] = 0.5f * (b[i].lo[j] + b[i].hi[j]);
+    }
+  }
+
+  void build() {
+    const uint32_t n = (uint32_t)boxes.size();
+    nodes.reserve(2 * (size_t)n / 2 + 8);
+    struct Task { int32_t node; uint32_t first, count, depth; };
+    std::vector<Task> work;
+    nodes.emplace_back();
+    work.push_back({0, 0, n, 0});
+    constexpr int NB = 32;
+    const double C_TRAV = 1.0, C_ISECT = 2.0;
+    while (!work.empty()) {
+      Task t = work.back();
+      work.pop_back();
+      max_depth = std::max(max_depth, t.depth);
+      Box bb, cb;
+      bb.empty();
+      cb.empty();
+      for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+        const uint32_t p = order[i];
+        bb.grow(boxes[p]);
+        Box c;
+        for (int j = 0; j < 3; ++j) c.lo[j] = c.hi[j] = cent[3 * p + j];
+        cb.grow(c);
+      }
+      nodes[t.node].box = bb;
+      auto make_leaf = [&]() {
+        nodes[t.node].left = -1;
+        nodes[t.node].first = t.first;
+        nodes[t.node].count = t.count;
+      };
+      if (t.count <= 1) { make_leaf(); continue; }
+      // best binned SAH split over the three axes
+      int best_axis = -1, best_bin = -1;
+      double best_cost = INFINITY;
+      if (t.depth < 48) {
+        for (int ax = 0; ax < 3; ++ax) {
+          const float ext = cb.hi[ax] - cb.lo[ax];
+          if (!(ext > 0.0f)) continue;
+          Box bin_box[NB];
+          uint32_t bin_cnt[NB] = {0};
+          for (int b = 0; b < NB; ++b) bin_box[b].empty();
+          const float scale = NB / ext;
+          for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+            const uint32_t p = order[i];
+            int b = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale);
+            b = std::min(std::max(b, 0), NB - 1);
+            bin_cnt[b]++;
+            bin_box[b].grow(boxes[p]);
+          }
+          double right_area[NB];
+          uint32_t right_cnt[NB];
+          Box acc;
+          acc.empty();
+          uint32_t cnt = 0;
+          for (int b = NB - 1; b > 0; --b) {
+            acc.grow(bin_box[b]);
+            cnt += bin_cnt[b];
+            right_area[b] = acc.area();
+            right_cnt[b] = cnt;
+          }
+          acc.empty();
+          cnt = 0;
+          for (int b = 0; b < NB - 1; ++b) {
+            acc.grow(bin_box[b]);
+            cnt += bin_cnt[b];
+            if (cnt == 0 || right_cnt[b + 1] == 0) continue;
+            const double c = acc.area() * cnt + right_area[b + 1] * right_cnt[b + 1];
+            if (c < best_cost) { best_cost = c; best_axis = ax; best_bin = b; }
+          }
+        }
+      }
+      const double parent_area = std::max(bb.area(), 1e-30);
+      const double split_cost = C_TRAV + C_ISECT * best_cost / parent_area;
+      const double leaf_cost = C_ISECT * t.count;
+      if (t.count <= GT_LEAF_MAX && (best_axis < 0 || leaf_cost <= split_cost)) { make_leaf(); continue; }
+      uint32_t mid;
+      if (best_axis >= 0) {
+        const int ax = best_axis;
+        const float ext = cb.hi[ax] - cb.lo[ax];
+        const float scale = NB / ext;
+        auto it = std::partition(order.begin() + t.first, order.begin() + t.first + t.count, [&](uint32_t p) {
+          int b = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale);
+          b = std::min(std::max(b, 0), NB - 1);
+          return b <= best_bin;
+        });
+        mid = (uint32_t)(it - order.begin());
+      } else {
+        mid = t.first + t.count;  // force the median fallback below
+      }
+      if (mid == t.first || mid == t.first + t.count) {
+        // degenerate or too deep: object median on the widest centroid axis
+        int ax = 0;
+        for (int j = 1; j < 3; ++j)
+          if (cb.hi[j] - cb.lo[j] > cb.hi[ax] - cb.lo[ax]) ax = j;
+        mid = t.first + t.count / 2;
+        std::nth_element(order.begin() + t.first, order.begin() + mid, order.begin() + t.first + t.count,
+                         [&](uint32_t a, uint32_t b) {
+                           if (cent[3 * a + ax] != cent[3 * b + ax]) return cent[3 * a + ax] < cent[3 * b + ax];
+                           return a < b;
+                         });
+      }
+      const int32_t l = (int32_t)nodes.size();
+      nodes.emplace_back();
+      const int32_t r = (int32_t)nodes.size();
+      nodes.emplace_back();
+      nodes[t.node].left = l;
+      nodes[t.node].right = r;
+      work.push_back({r, mid, t.first + t.count - mid, t.depth + 1});
+      work.push_back({l, t.first, mid - t.first, t.depth + 1});
+    }
+  }
+};
+
+/* Collapse the binary tree into 4-wide nodes: repeatedly open the inner
+ * child with the largest surface area until four children are present. */
+struct Collapser {
+  const std::vector<B2Node>& b2;
+  std::vector<gt_node4> out;
+  uint32_t depth = 0;
+
+  explicit Collapser(const std::vector<B2Node>& n) : b2(n) {}
+
+  static uint32_t leaf_of(const B2Node& n) { return gt_leaf_ref(n.first, n.count); }
+
+  // returns (node index, stack need)
+  std::pair<uint32_t, uint32_t> emit(int32_t b2i, uint32_t d) {
+    depth = std::max(depth, d + 1);
+    std::vector<int32_t> ch;
+    const B2Node& root = b2[b2i];
+    if (root.left < 0) {
+      ch.push_back(b2i);
+    } else {
+      ch.push_back(root.left);
+      ch.push_back(root.right);
+      while (ch.size() < 4) {
+        int best = -1;
+        double best_area = -1.0;
+        for (size_t i = 0; i < ch.size(); ++i) {
+          const B2Node& c = b2[ch[i]];
+          if (c.left < 0) continue;
+          const double a = c.box.area();
+          if (a > best_area) { best_area = a; best = (int)i; }
+        }
+        if (best < 0) break;
+        const int32_t open = ch[best];
+        ch[best] = b2[open].left;
+        ch.insert(ch.begin() + best + 1, b2[open].right);
+      }
+    }
+    const uint32_t me = (uint32_t)out.size();
+    out.emplace_back();
+    gt_node4 node;
+    std::memset(&node, 0, sizeof(node));
+    uint32_t child_need = 0;
+    for (int i = 0; i < 4; ++i) {
+      node.lo_x[i] = node.lo_y[i] = node.lo_z[i] = INFINITY;  // empty slots never hit
+      node.hi_x[i] = node.hi_y[i] = node.hi_z[i] = -INFINITY;
+      node.child[i] = GT_NODE_EMPTY;
+    }
+    for (size_t i = 0; i < ch.size(); ++i) {
+      const B2Node& c = b2[ch[i]];
+      node.lo_x[i] = c.box.lo[0]; node.hi_x[i] = c.box.hi[0];
+      node.lo_y[i] = c.box.lo[1]; node.hi_y[i] = c.box.hi[1];
+      node.lo_z[i] = c.box.lo[2]; node.hi_z[i] = c.box.hi[2];
+      if (c.left < 0) {
+        node.child[i] = leaf_of(c);
+      } else {
+        auto sub = emit(ch[i], d + 1);
+        node.child[i] = sub.first;
+        child_need = std::max(child_need, sub.second);
+      }
+    }
+    out[me] = node;
+    const uint32_t m = (uint32_t)ch.size();
+    const uint32_t need = std::max(m, (m - 1) + child_need);
+    return {me, need};
+  }
+};
+
+}  // namespace
+
+static void face_table(float* f) {
+  for (int i = 0; i < GT_NUM_FACES; ++i) {
+    const gt_face fc = gt_ico_face(i);
+    const float v[12] = {fc.p0.x, fc.p0.y, fc.p0.z, fc.e0.x, fc.e0.y, fc.e0.z,
+                         fc.e1.x, fc.e1.y, fc.e1.z, fc.n.x,  fc.n.y,  fc.n.z};
+    std::memcpy(f + 12 * i, v, sizeof(v));
+  }
+}
+
+template <typename T>
+static int upload(DevBuf& b, const T* src, size_t count) {
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+  b.bytes = std::max<size_t>(count * sizeof(T), 256);
+  hipError_t e = hipMalloc(&b.p, b.bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    gt_set_error("hipMalloc(%zu) failed: %s", b.bytes, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? GT_E_NOMEM : GT_E_HIP;
+  }
+  if (count) HIP_TRY(hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return GT_OK;
+}
+
+
+static int build_and_upload(gt_scene* s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  HostScene& h = s->h;
+  build_proxies(h, s->alpha_min);
+  const uint32_t np = (uint32_t)h.proxy_ids.size();
+  if (np > GT_MAX_PROXIES) {
+    gt_set_error("%u proxies exceed the limit %u", np, GT_MAX_PROXIES);
+    return GT_E_LIMIT;
+  }
+  std::vector<Box> boxes(np);
+  for (uint32_t i = 0; i < np; ++i) boxes[i] = proxy_box(&h.o2w[(size_t)12 * i]);
+
+  std::vector<gt_node4> nodes;
+  std::vector<float> prox((size_t)np * 16, 0.0f);
+  uint32_t root = GT_NODE_EMPTY, need = 0, depth = 0;
+  if (np > 0) {
+    Builder b(boxes);
+    b.build();
+    Collapser c(b.nodes);
+    auto r = c.emit(0, 0);
+    root = r.first;
+    need = r.second;
+    depth = c.depth;
+    nodes.swap(c.out);
+    for (uint32_t i = 0; i < np; ++i) {
+      const uint32_t p = b.order[i];
+      float* d = &prox[(size_t)16 * i];
+      std::memcpy(d, &h.w2o[(size_t)12 * p], 12 * sizeof(float));
+      const uint32_t gid = h.proxy_ids[p];
+      std::memcpy(d + 12, &gid, 4);
+    }
+  }
+  if (need > GT_STACK_MAX) {
+    gt_set_error("BVH needs a traversal stack of %u entries (limit %u)", need, GT_STACK_MAX);
+    return GT_E_LIMIT;
+  }
+  // per-Gaussian shading records
+  std::vector<float> geom((size_t)h.n * 12, 0.0f), sh((size_t)h.n * 48, 0.0f);
+  for (uint32_t i = 0; i < h.n; ++i) {
+    const float* o = &h.packed[(size_t)i * 80];
+    float* g = &geom[(size_t)i * 12];
+    g[0] = o[0]; g[1] = o[1]; g[2] = o[2]; g[3] = o[12];
+    gt_inv_cov(o + 4, o + 8, g + 4);
+    for (int j = 0; j < 16; ++j)
+      for (int c = 0; c < 3; ++c) sh[(size_t)i * 48 + 3 * j + c] = o[16 + 4 * j + c];
+  }
+  float faces[12 * GT_NUM_FACES];
+  face_table(faces);
+  int rc;
+  if ((rc = upload(s->geom, geom.data(), geom.size())) != GT_OK) return rc;
+  if ((rc = upload(s->sh, sh.data(), sh.size())) != GT_OK) return rc;
+  if ((rc = upload(s->prox, prox.data(), prox.size())) != GT_OK) return rc;
+  if ((rc = upload(s->nodes, nodes.data(), nodes.size())) != GT_OK) return rc;
+  if ((rc = upload(s->faces, faces, 12 * GT_NUM_FACES)) != GT_OK) return rc;
+  s->dev.geom = (const float*)s->geom.p;
+  s->dev.sh = (const float*)s->sh.p;
+  s->dev.prox = (const float*)s->prox.p;
+  s->dev.nodes = (const gt_node4*)s->nodes.p;
+  s->dev.faces = (const float*)s->faces.p;
+  s->dev.root = root;
+  s->dev.n = h.n;
+  s->dev.n_prox = np;
+  s->dev.n_nodes = (uint32_t)nodes.size();
+  s->n_nodes = (uint32_t)nodes.size();
+  s->max_stack = need;
+  s->depth = depth;
+  s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GT_OK;
+}
+
+static int finish_create(std::unique_ptr<gt_scene>& s, gt_scene** out) {
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    gt_set_error("no HIP device available");
+    return GT_E_HIP;
+  }
+  const int rc = build_and_upload(s.get());
+  if (rc != GT_OK) return rc;
+  *out = s.release();
+  return GT_OK;
+}
+
+extern "C" int gt_scene_create_raw(const float* raw, uint32_t n, double alpha_min, gt_scene** out) {
+  if (!out) { gt_set_error("gt_scene_create_raw: out is NULL"); return GT_E_INVALID; }
+  *out = nullptr;
+  if (n && !raw) { gt_set_error("gt_scene_create_raw: raw is NULL"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_scene_create_raw: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  std::unique_ptr<gt_scene> s(new (std::nothrow) gt_scene());
+  if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  try {
+    s->h.n = n;
+    s->h.raw.assign(raw, raw + (size_t)n * GT_RAW_FLOATS);
+    s->alpha_min = alpha_min;
+    activate_raw(s->h);
+    return finish_create(s, out);
+  } catch (const std::bad_alloc&) {
+    gt_set_error("out of host memory");
+    return GT_E_NOMEM;
+  }
+}
+
+extern "C" int gt_scene_create_packed(const void* records, uint32_t n, double alpha_min, gt_scene** out) {
+  if (!out) { gt_set_error("gt_scene_create_packed: out is NULL"); return GT_E_INVALID; }
+  *out = nullptr;
+  if (n && !records) { gt_set_error("gt_scene_create_packed: records is NULL"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_scene_create_packed: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  std::unique_ptr<gt_scene> s(new (std::nothrow) gt_scene());
+  if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  try {
+    s->h.n = n;
+    const float* f = (const float*)records;
+    s->h.packed.assign(f, f + (size_t)n * 80);
+    s->alpha_min = alpha_min;
+    activate_packed(s->h);
+    return finish_create(s, out);
+  } catch (const std::bad_alloc&) {
+    gt_set_error("out of host memory");
+    return GT_E_NOMEM;
+  }
+}
+
+extern "C" int gt_scene_set_alpha_min(gt_scene* s, double alpha_min) {
+  if (!s) { gt_set_error("gt_scene_set_alpha_min: scene is NULL"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_scene_set_alpha_min: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  s->alpha_min = alpha_min;
+  try {
+    return build_and_upload(s);
+  } catch (const std::bad_alloc&) {
+    gt_set_error("out of host memory");
+    return GT_E_NOMEM;
+  }
+}
+
+extern "C" void gt_scene_destroy(gt_scene* s) { delete s; }
+
+extern "C" int gt_scene_get_info(const gt_scene* s, gt_scene_info* info) {
+  if (!s || !info) { gt_set_error("gt_scene_get_info: NULL argument"); return GT_E_INVALID; }
+  std::memset(info, 0, sizeof(*info));
+  info->n_gaussians = s->h.n;
+  info->n_proxies = s->dev.n_prox;
+  info->n_nodes = s->n_nodes;
+  info->max_stack = s->max_stack;
+  info->bvh_depth = s->depth;
+  info->device_bytes = s->geom.bytes + s->sh.bytes + s->prox.bytes + s->nodes.bytes + s->faces.bytes;
+  info->build_ms = s->build_ms;
+  return GT_OK;
+}
+
+extern "C" int gt_scene_get_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w,
+                                      float* w2o) {
+  if (!s) { gt_set_error("gt_scene_get_host_data: scene is NULL"); return GT_E_INVALID; }
+  const HostScene& h = s->h;
+  if (packed) std::memcpy(packed, h.packed.data(), h.packed.size() * sizeof(float));
+  if (proxy_ids) std::memcpy(proxy_ids, h.proxy_ids.data(), h.proxy_ids.size() * sizeof(uint32_t));
+  if (o2w) std::memcpy(o2w, h.o2w.data(), h.o2w.size() * sizeof(float));
+  if (w2o) std::memcpy(w2o, h.w2o.data(), h.w2o.size() * sizeof(float));
+  return GT_OK;
+}
+
+/* ---------------- render ---------------- */
+int gt_validate_uniforms(const gt_uniforms* u, uint32_t width, uint32_t height) {
+  if (!u) { gt_set_error("uniforms are NULL"); return GT_E_INVALID; }
+  if (width < 1 || height < 1 || width > 16384 || height > 16384) {
+    gt_set_error("width(%u), height(%u) must be in 1..16384", width, height);
+    return GT_E_INVALID;
+  }
+  if (u->k < 1 || u->k > GT_MAX_K) {
+    gt_set_error("hit_array_size k=%u must be in 1..%d", u->k, GT_MAX_K);
+    return GT_E_LIMIT;
+  }
+  if (!(u->t_min >= 0.0f) || !std::isfinite(u->t_min) || !(u->t_max > u->t_min)) {
+    gt_set_error("need 0 <= t_min < t_max (t_min=%g, t_max=%g)", u->t_min, u->t_max);
+    return GT_E_INVALID;
+  }
+  if (u->sh_degree_max > 3) { gt_set_error("sh_degree_max %u > 3", u->sh_degree_max); return GT_E_INVALID; }
+  if (u->debug_mode > 2) { gt_set_error("debugMode %u not in {0,1,2}", u->debug_mode); return GT_E_INVALID; }
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(u->to_world[i])) { gt_set_error("to_world has a non-finite entry"); return GT_E_INVALID; }
+  if (!std::isfinite(u->camera_aspect[0]) || !std::isfinite(u->camera_aspect[1]) || !std::isfinite(u->T_min) ||
+      !std::isfinite(u->alpha_min)) {
+    gt_set_error("non-finite uniform value");
+    return GT_E_INVALID;
+  }
+  return GT_OK;
+}
+
+void gt_fill_dev_frame(const gt_uniforms* u, uint32_t width, uint32_t height, gt_dev_frame* f) {
+  std::memset(f, 0, sizeof(*f));
+  std::memcpy(f->to_world, u->to_world, sizeof(f->to_world));
+  f->ax = u->camera_aspect[0];
+  f->ay = u->camera_aspect[1];
+  f->t_min = u->t_min;
+  f->t_max = u->t_max;
+  f->T_min = u->T_min;
+  f->alpha_min = u->alpha_min;
+  f->k = u->k;
+  f->sh_degree = u->sh_degree_max;
+  f->early_stop = u->early_stop;
+  f->debug_mode = u->debug_mode;
+  f->width = width;
+  f->height = height;
+  const float om = std::max(std::fabs(u->to_world[12]), std::max(std::fabs(u->to_world[13]), std::fabs(u->to_world[14])));
+  f->slack_abs = 2e-5f * (om + 1.0f);
+}
+
+const gt_dev_scene* gt_scene_dev(const gt_scene* s) { return &s->dev; }
+
+extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
+                         int buf_flags, void* stream) {
+  if (!s || !out || !out->rgba) { gt_set_error("gt_render: scene, out and out->rgba are required"); return GT_E_INVALID; }
+  int rc = gt_validate_uniforms(u, width, height);
+  if (rc != GT_OK) return rc;
+  if ((out->hit_ids || out->accept) && out->max_rounds == 0) {
+    gt_set_error("gt_render: max_rounds must be >= 1 when hit_ids/accept are requested");
+    return GT_E_INVALID;
+  }
+  gt_dev_frame f;
+  gt_fill_dev_frame(u, width, height, &f);
+  const size_t R = (size_t)width * height;
+  const size_t MR = out->max_rounds;
+  if (buf_flags == GT_BUF_DEVICE) {
+    f.rgba = out->rgba;
+    f.hit_ids = out->hit_ids;
+    f.accept = out->accept;
+    f.n_rounds = out->n_rounds;
+    f.T_out = out->T;
+    f.stats = (unsigned long long*)out->stats;
+    f.max_rounds = (uint32_t)MR;
+    return gt_launch_forward(&s->dev, &f, stream);
+  }
+  // host buffers: device temporaries, synchronous
+  struct Tmp {
+    void* p = nullptr;
+    ~Tmp() { if (p) (void)hipFree(p); }
+  } t_rgba, t_hits, t_acc, t_rounds, t_T, t_stats;
+  auto alloc = [&](Tmp& t, size_t bytes) -> int {
+    hipError_t e = hipMalloc(&t.p, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) {
+      t.p = nullptr;
+      gt_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+      return GT_E_NOMEM;
+    }
+    return GT_OK;
+  };
+  if ((rc = alloc(t_rgba, R * 16)) != GT_OK) return rc;
+  f.rgba = (float*)t_rgba.p;
+  if (out->hit_ids) {
+    if ((rc = alloc(t_hits, R * MR * u->k * 4)) != GT_OK) return rc;
+    f.hit_ids = (uint32_t*)t_hits.p;
+  }
+  if (out->accept) {
+    if ((rc = alloc(t_acc, R * MR * 4)) != GT_OK) return rc;
+    f.accept = (uint32_t*)t_acc.p;
+  }
+  if (out->n_rounds) {
+    if ((rc = alloc(t_rounds, R * 4)) != GT_OK) return rc;
+    f.n_rounds = (uint32_t*)t_rounds.p;
+  }
+  if (out->T) {
+    if ((rc = alloc(t_T, R * 4)) != GT_OK) return rc;
+    f.T_out = (float*)t_T.p;
+  }
+  if (out->stats) {
+    if ((rc = alloc(t_stats, GT_NSTATS * 8)) != GT_OK) return rc;
+    HIP_TRY(hipMemset(t_stats.p, 0, GT_NSTATS * 8));
+    f.stats = (unsigned long long*)t_stats.p;
+  }
+  f.max_rounds = (uint32_t)MR;
+  if ((rc = gt_launch_forward(&s->dev, &f, nullptr)) != GT_OK) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(out->rgba, f.rgba, R * 16, hipMemcpyDeviceToHost));
+  if (out->hit_ids) HIP_TRY(hipMemcpy(out->hit_ids, f.hit_ids, R * MR * u->k * 4, hipMemcpyDeviceToHost));
+  if (out->accept) HIP_TRY(hipMemcpy(out->accept, f.accept, R * MR * 4, hipMemcpyDeviceToHost));
+  if (out->n_rounds) HIP_TRY(hipMemcpy(out->n_rounds, f.n_rounds, R * 4, hipMemcpyDeviceToHost));
+  if (out->T) HIP_TRY(hipMemcpy(out->T, f.T_out, R * 4, hipMemcpyDeviceToHost));
+  if (out->stats) HIP_TRY(hipMemcpy(out->stats, f.stats, GT_NSTATS * 8, hipMemcpyDeviceToHost));
+  return GT_OK;
+}
+
+/* Host stage of gt_scene_create_raw without a device: activations, packing
+ * and proxy construction.  Outputs sized for n (packed n x 80, ids n,
+ * o2w/w2o n x 12); *n_proxies receives the count. */
+extern "C" int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, float* packed, uint32_t* proxy_ids,
+                               float* o2w, float* w2o, uint32_t* n_proxies) {
+  if ((n && !raw) || !n_proxies) { gt_set_error("gt_host_prepare: NULL argument"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_host_prepare: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  try {
+    HostScene h;
+    h.n = n;
+    h.raw.assign(raw, raw + (size_t)n * GT_RAW_FLOATS);
+    activate_raw(h);
+    build_proxies(h, alpha_min);
+    if (packed) std::memcpy(packed, h.packed.data(), h.packed.size() * sizeof(float));
+    if (proxy_ids) std::memcpy(proxy_ids, h.proxy_ids.data(), h.proxy_ids.size() * sizeof(uint32_t));
+    if (o2w) std::memcpy(o2w, h.o2w.data(), h.o2w.size() * sizeof(float));
+    if (w2o) std::memcpy(w2o, h.w2o.data(), h.w2o.size() * sizeof(float));
+    *n_proxies = (uint32_t)h.proxy_ids.size();
+    return GT_OK;
+  } catch (const std::bad_alloc&) {
+    gt_set_error("out of host memory");
+    return GT_E_NOMEM;
+  }
+}

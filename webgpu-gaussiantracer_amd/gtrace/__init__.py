@@ -1,0 +1,289 @@
+"""Python binding of libgt.so (the MI355X Gaussian ray tracer C-ABI).
+
+Mirrors the reference's per-frame surface
+    render(camera, {alpha_min, hit_array_size, T_min, debugMode}) -> RGBA
+(src/main.ts:448-534, 542-600) with
+    Scene            the acceleration container (gaussianSceneToBvh, main.ts:30-111)
+    Uniforms         the 112-B uniform block (GS_shader.ts:96-108, main.ts:520-534)
+    Scene.render()   traceRays(W, H) + readback (pass_encoder.ts:35-65)
+The library must be built (webgpu-gaussiantracer_amd/build_lib.py); there is
+no fallback path: a missing libgt.so raises at import.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgt.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libgt.so not built at {LIB_PATH}; run __graft_entry__.build()")
+_lib = C.CDLL(LIB_PATH)
+
+GT_OK = 0
+GT_RAW_FLOATS = 59
+GT_MAX_K = 32
+GT_NSTATS = 8
+SENTINEL = 0xFFFFFFFF
+STAT_NAMES = ["rays", "rounds", "h_eval", "h_comp", "proxies", "hits", "nodes", "overflow"]
+
+
+class GTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gt error {code}: {msg}")
+        self.code = code
+
+
+class Uniforms(C.Structure):
+    _fields_ = [
+        ("to_world", C.c_float * 16), ("camera_aspect", C.c_float * 2),
+        ("t_min", C.c_float), ("t_max", C.c_float), ("T_min", C.c_float), ("alpha_min", C.c_float),
+        ("k", C.c_uint32), ("sh_degree_max", C.c_uint32), ("accumulated_frames", C.c_uint32),
+        ("early_stop", C.c_uint32), ("debug_mode", C.c_uint32), ("_pad", C.c_uint32),
+    ]
+
+
+assert C.sizeof(Uniforms) == 112
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("rgba", C.c_void_p), ("hit_ids", C.c_void_p), ("accept", C.c_void_p),
+        ("n_rounds", C.c_void_p), ("T", C.c_void_p), ("stats", C.c_void_p),
+        ("max_rounds", C.c_uint32), ("_pad", C.c_uint32),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("n_gaussians", C.c_uint32), ("n_proxies", C.c_uint32), ("n_nodes", C.c_uint32),
+        ("max_stack", C.c_uint32), ("bvh_depth", C.c_uint32), ("_pad", C.c_uint32),
+        ("device_bytes", C.c_uint64), ("build_ms", C.c_double),
+    ]
+
+
+class GenParams(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32), ("n_clusters", C.c_uint32), ("box_half", C.c_float),
+        ("cluster_sigma", C.c_float), ("log_scale_mean", C.c_float), ("log_scale_std", C.c_float),
+        ("logit_mean", C.c_float), ("logit_std", C.c_float), ("dc_std", C.c_float),
+        ("rest_std", C.c_float), ("seed", C.c_uint64),
+    ]
+
+
+_P = C.c_void_p
+_lib.gt_last_error.restype = C.c_char_p
+_lib.gt_version.restype = C.c_char_p
+_lib.gt_uniforms_init.argtypes = [C.POINTER(Uniforms), C.POINTER(C.c_float), C.c_double, C.c_uint32, C.c_uint32,
+                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+                                  C.c_int32, C.c_uint32]
+_lib.gt_camera_look_at.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+_lib.gt_scene_create_raw.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
+_lib.gt_scene_create_packed.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
+_lib.gt_scene_set_alpha_min.argtypes = [_P, C.c_double]
+_lib.gt_scene_destroy.argtypes = [_P]
+_lib.gt_scene_destroy.restype = None
+_lib.gt_scene_get_info.argtypes = [_P, C.POINTER(SceneInfo)]
+_lib.gt_scene_get_host_data.argtypes = [_P, _P, _P, _P, _P]
+_lib.gt_host_prepare.argtypes = [_P, C.c_uint32, C.c_double, _P, _P, _P, _P, C.POINTER(C.c_uint32)]
+_lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
+_lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
+_lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                             C.POINTER(C.c_uint32)]
+_lib.gt_free.argtypes = [_P]
+_lib.gt_free.restype = None
+
+EXPORTED = [
+    "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
+    "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
+    "gt_render", "gt_host_prepare", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
+]
+
+
+def lib():
+    return _lib
+
+
+def _check(rc):
+    if rc != GT_OK:
+        raise GTError(rc, _lib.gt_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def last_error():
+    return _lib.gt_last_error().decode()
+
+
+def camera_look_at(pos, target):
+    """WASDCamera matrix (camera.ts) looking from pos at target, column-major f32[16]."""
+    p = (C.c_float * 3)(*pos)
+    t = (C.c_float * 3)(*target)
+    m = (C.c_float * 16)()
+    _lib.gt_camera_look_at(p, t, m)
+    return np.array(m[:], dtype=np.float32)
+
+
+def make_uniforms(to_world, width, height, hfov_deg=60.0, t_min=1e-3, t_max=1e5, T_min=0.03,
+                  alpha_min=0.2, k=6, sh_degree_max=3, early_stop=0, debug_mode=0):
+    """upadateUniformData (src/main.ts:520-534) with the reference's defaults (448-464)."""
+    u = Uniforms()
+    tw = (C.c_float * 16)(*[float(x) for x in np.asarray(to_world, dtype=np.float32).ravel()])
+    _lib.gt_uniforms_init(C.byref(u), tw, float(hfov_deg), width, height, t_min, t_max, T_min, alpha_min,
+                          k, sh_degree_max, early_stop, debug_mode)
+    return u
+
+
+def uniforms_as_dict(u):
+    return {
+        "to_world": np.array(u.to_world[:], dtype=np.float32), "aspect": (u.camera_aspect[0], u.camera_aspect[1]),
+        "t_min": u.t_min, "t_max": u.t_max, "T_min": u.T_min, "alpha_min": u.alpha_min, "k": u.k,
+        "sh_degree_max": u.sh_degree_max, "early_stop": u.early_stop, "debug_mode": u.debug_mode,
+    }
+
+
+# Seeded synthetic scenes of the benchmark configurations
+CONFIGS = {
+    # 20k Gaussians uniform in a 4^3 box, log-scale N(-3, 0.5), logit N(0, 1.5)
+    "c1": dict(n=20000, n_clusters=0, box_half=2.0, cluster_sigma=0.0, log_scale_mean=-3.0,
+               log_scale_std=0.5, logit_mean=0.0, logit_std=1.5),
+    # 1M Gaussians, 64 clusters in a 20^3 box, offsets N(0, 0.8), log-scale N(-4, 0.6), logit N(0.5, 1.5)
+    "c2": dict(n=1000000, n_clusters=64, box_half=10.0, cluster_sigma=0.8, log_scale_mean=-4.0,
+               log_scale_std=0.6, logit_mean=0.5, logit_std=1.5),
+    # 200k Gaussians, one cluster N(0, 0.8) at the origin, log-scale N(-3.5, 0.5)
+    "c4": dict(n=200000, n_clusters=1, box_half=0.0, cluster_sigma=0.8, log_scale_mean=-3.5,
+               log_scale_std=0.5, logit_mean=0.5, logit_std=1.5),
+    # 3M Gaussians, 256 clusters in a 40^3 box, log-scale N(-4.5, 0.7)
+    "c5": dict(n=3000000, n_clusters=256, box_half=20.0, cluster_sigma=0.8, log_scale_mean=-4.5,
+               log_scale_std=0.7, logit_mean=0.5, logit_std=1.5),
+}
+
+
+def generate_raw(n=None, seed=0, config=None, **kw):
+    """PLY-raw parameters (n x 59 float32) from the library's seeded generator."""
+    d = dict(n_clusters=0, box_half=2.0, cluster_sigma=0.8, log_scale_mean=-3.0, log_scale_std=0.5,
+             logit_mean=0.0, logit_std=1.5, dc_std=0.5, rest_std=0.05)
+    if config is not None:
+        d.update(CONFIGS[config])
+    d.update(kw)
+    if n is not None:
+        d["n"] = n
+    g = GenParams(seed=seed, **d)
+    raw = np.zeros((g.n, GT_RAW_FLOATS), dtype=np.float32)
+    _check(_lib.gt_generate_scene(C.byref(g), _ptr(raw)))
+    return raw
+
+
+def read_ply(path):
+    """PLY-raw records of a 3DGS .ply (GS_ply.ts semantics) and its SH degree."""
+    p = C.POINTER(C.c_float)()
+    n = C.c_uint32()
+    deg = C.c_uint32()
+    _check(_lib.gt_ply_read(path.encode(), C.byref(p), C.byref(n), C.byref(deg)))
+    try:
+        raw = np.ctypeslib.as_array(p, shape=(n.value * GT_RAW_FLOATS,)).copy().reshape(n.value, GT_RAW_FLOATS)
+    finally:
+        _lib.gt_free(C.cast(p, C.c_void_p))
+    return raw, deg.value
+
+
+def host_prepare(raw, alpha_min):
+    """Host stage of scene creation (no GPU needed): packed records and proxies."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    n = raw.shape[0]
+    packed = np.zeros((n, 80), np.float32)
+    ids = np.zeros(n, np.uint32)
+    o2w = np.zeros((n, 12), np.float32)
+    w2o = np.zeros((n, 12), np.float32)
+    npx = C.c_uint32()
+    _check(_lib.gt_host_prepare(_ptr(raw), n, float(alpha_min), _ptr(packed), _ptr(ids), _ptr(o2w), _ptr(w2o),
+                                C.byref(npx)))
+    m = npx.value
+    return dict(packed=packed, proxy_ids=ids[:m].copy(), o2w=o2w[:m].copy(), w2o=w2o[:m].copy())
+
+
+class Scene:
+    """Acceleration container over the Gaussians (gaussianSceneToBvh, src/main.ts:30-111)."""
+
+    def __init__(self, raw=None, alpha_min=0.2, packed=None):
+        self._h = C.c_void_p()
+        if raw is not None:
+            raw = np.ascontiguousarray(raw, dtype=np.float32)
+            assert raw.ndim == 2 and raw.shape[1] == GT_RAW_FLOATS
+            _check(_lib.gt_scene_create_raw(_ptr(raw), raw.shape[0], float(alpha_min), C.byref(self._h)))
+        elif packed is not None:
+            packed = np.ascontiguousarray(packed, dtype=np.float32).reshape(-1, 80)
+            _check(_lib.gt_scene_create_packed(_ptr(packed), packed.shape[0], float(alpha_min), C.byref(self._h)))
+        else:
+            raise ValueError("raw or packed is required")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.gt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_alpha_min(self, alpha_min):
+        _check(_lib.gt_scene_set_alpha_min(self._h, float(alpha_min)))
+
+    def info(self):
+        i = SceneInfo()
+        _check(_lib.gt_scene_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in SceneInfo._fields_ if not f.startswith("_")}
+
+    def host_data(self):
+        inf = self.info()
+        n, npx = inf["n_gaussians"], inf["n_proxies"]
+        packed = np.zeros((n, 80), np.float32)
+        ids = np.zeros(npx, np.uint32)
+        o2w = np.zeros((npx, 12), np.float32)
+        w2o = np.zeros((npx, 12), np.float32)
+        _check(_lib.gt_scene_get_host_data(self._h, _ptr(packed), _ptr(ids), _ptr(o2w), _ptr(w2o)))
+        return dict(packed=packed, proxy_ids=ids, o2w=o2w, w2o=w2o)
+
+    def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False):
+        """traceRays(W, H) with host buffers: returns rgba (H, W, 4) and, on request,
+        hit lists (H*W, max_rounds, k), accept masks, round counts, T and stats."""
+        R = width * height
+        rgba = np.zeros((height, width, 4), np.float32)
+        out = {"rgba": rgba}
+        f = Frame()
+        f.rgba = _ptr(rgba)
+        if max_rounds:
+            hits = np.zeros((R, max_rounds, u.k), np.uint32)
+            acc = np.zeros((R, max_rounds), np.uint32)
+            rounds = np.zeros(R, np.uint32)
+            f.hit_ids, f.accept, f.n_rounds = _ptr(hits), _ptr(acc), _ptr(rounds)
+            f.max_rounds = max_rounds
+            out.update(hit_ids=hits, accept=acc, n_rounds=rounds)
+        if want_T or max_rounds:
+            T = np.zeros(R, np.float32)
+            f.T = _ptr(T)
+            out["T"] = T
+        if want_stats:
+            st = np.zeros(GT_NSTATS, np.uint64)
+            f.stats = _ptr(st)
+            out["stats"] = st
+        _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 0, None))
+        if want_stats:
+            out["stats"] = dict(zip(STAT_NAMES, [int(x) for x in out["stats"]]))
+        return out
+
+    def render_device(self, u, width, height, rgba_ptr, stream_ptr=None, stats_ptr=None):
+        """Asynchronous traceRays into caller-owned device memory (GT_BUF_DEVICE)."""
+        f = Frame()
+        f.rgba = rgba_ptr
+        f.stats = stats_ptr
+        _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 1, stream_ptr))
