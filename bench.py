@@ -49,8 +49,20 @@ def main():
     ap.add_argument("--n", type=int, default=1000000)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="time 1/N of the rays on the CPU")
+    ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(rays shard with no exchange; see the sharding notes in DESIGN.md)", file=sys.stderr)
@@ -147,14 +159,15 @@ def main():
         o.prebuild_bvh()
         p = orc.params_from_uniforms(u, W, H)
         rng = np.random.Generator(np.random.PCG64(2))
-        n_s = max(1, R // args.cpu_sample)
+        n_s = max(1, R // args.cpu_sample)  # a seeded 1/N sample of the frame
         rays = np.sort(rng.choice(R, n_s, replace=False)).astype(np.uint32)
         c0 = time.perf_counter()
         rr = orc.render(o, p, threads=threads, ray_list=rays)
         cs = time.perf_counter() - c0
         out["cpu_baseline"] = {
             "value": round(n_s / cs / 1e6, 5), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{n_s} seeded random rays (1/{args.cpu_sample} of the frame), oracle BVH variant, "
+            "sample": f"{n_s} seeded random rays (1/{args.cpu_sample} of the frame) of the same workload, "
+                      f"oracle BVH variant (CPU restatement of the reference path, not the reference binary), "
                       f"{cs:.2f} s",
             "h_eval_per_ray": rr["stats"]["h_eval"] / n_s,
         }

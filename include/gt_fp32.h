@@ -14,12 +14,12 @@
  *   object-space ray        webrtx/src/glsl/trace.glsl:102-105
  *   triangle test           webrtx/src/glsl/intersect.glsl:73-92
  *   facing / back-face cull webrtx/src/glsl/trace.glsl:153-164
- *   Gaussian response       src/GS_shader.ts:111-136
+ *   Gaussian response       src/GS_shader.ts:111-136 (evaluated in FP64, below)
  *   relaunch epsilon        src/GS_shader.ts:197, 248
  *   icosahedron proxy       src/main.ts:12-25
  * GLSL/WGSL leave dot-product summation order and exp accuracy to the
  * implementation; the orders fixed below ((x+y)+z, columns left to right)
- * and gt_expf are this library's definition of them.
+ * and gt_exp64 are this library's definition of them.
  *
  * Plain C99 + HIP.  Include from C, C++ or .hip translation units.
  */
@@ -84,53 +84,58 @@ GT_HD uint32_t gt_f32_to_bits(float f) {
 #endif
 }
 
-GT_HD float gt_rintf(float x) {
+GT_HD double gt_rint64(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_rintf(x);
+  return __builtin_rint(x);
 #else
-  return rintf(x); /* round-to-nearest-even in the default rounding mode */
+  return rint(x); /* round-to-nearest-even in the default rounding mode */
+#endif
+}
+GT_HD double gt_bits_to_f64(uint64_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __longlong_as_double((long long)u);
+#else
+  double f; memcpy(&f, &u, 8); return f;
 #endif
 }
 
 /*
- * gt_expf: exp(x) in FP32 by Cody-Waite range reduction followed by a
- * degree-7 Taylor polynomial in Horner form (the classic scheme of
- * W. J. Cody and W. Waite, "Software Manual for the Elementary Functions",
- * 1980, ch. 6).
+ * gt_exp64: exp(x) in FP64 by Cody-Waite range reduction followed by a
+ * degree-13 Taylor polynomial in Horner form (W. J. Cody and W. Waite,
+ * "Software Manual for the Elementary Functions", 1980, ch. 6):
  *   n  = rint(x * log2(e))
- *   r  = (x - n*LN2_HI) - n*LN2_LO     LN2_HI has 16 significant bits, so
- *                                        n*LN2_HI is exact for |n| < 2^8
- *   e^r = 1 + r + r^2/2! + ... + r^7/7! on |r| <= ln2/2 (truncation < 8e-9 rel.)
- *   exp(x) = e^r * 2^n, with 2^n built from its exponent bits.
- * x <= -86 returns 0 (the result would be within a factor 2 of the smallest
- * normal; returning 0 keeps every result normal or zero so flush-to-zero
- * settings cannot change it), x >= 88.5 returns +inf, NaN returns 0.
- * Accuracy: within 2 ulp of exp over (-86, 88.5).
+ *   r  = (x - n*LN2_HI) - n*LN2_LO     LN2_HI has 32 significant bits, so
+ *                                        n*LN2_HI is exact for |n| < 2^20
+ *   e^r = sum_{i<=13} r^i / i!  on |r| <= ln2/2   (truncation < 5e-18 rel.)
+ *   exp(x) = e^r * 2^n, 2^n built from its exponent bits.
+ * x <= -700 (and NaN) returns 0, x >= 700 returns +inf.  Relative error
+ * below 2e-15 on (-700, 700).
  */
-#define GT_EXP_LOG2E 1.44269504088896341f
-#define GT_EXP_LN2_HI 0.693145751953125f           /* 0x3f317200 */
-#define GT_EXP_LN2_LO 1.42860682030941723212e-06f  /* ln2 - LN2_HI */
-GT_HD float gt_expf(float x) {
-  if (!(x > -86.0f)) return 0.0f;
-  if (x >= 88.5f) return gt_bits_to_f32(0x7f800000u);
-  float fn = gt_rintf(x * GT_EXP_LOG2E);
+#define GT_EXP64_LOG2E 1.4426950408889634074
+#define GT_EXP64_LN2_HI 6.93147180369123816490e-01  /* 0x3fe62e42fee00000 */
+#define GT_EXP64_LN2_LO 1.90821492927058770002e-10  /* ln2 - LN2_HI */
+GT_HD double gt_exp64(double x) {
+  if (!(x > -700.0)) return 0.0;
+  if (x >= 700.0) return gt_bits_to_f64(0x7ff0000000000000ull);
+  double fn = gt_rint64(x * GT_EXP64_LOG2E);
   int n = (int)fn;
-  float r = x - fn * GT_EXP_LN2_HI;
-  r = r - fn * GT_EXP_LN2_LO;
-  float p = 1.0f / 5040.0f;
-  p = p * r + 1.0f / 720.0f;
-  p = p * r + 1.0f / 120.0f;
-  p = p * r + 1.0f / 24.0f;
-  p = p * r + 1.0f / 6.0f;
-  p = p * r + 0.5f;
-  p = p * r + 1.0f;
-  p = p * r + 1.0f;
-  /* n in [-124, 128]; split 2^n in two factors so both stay normal */
-  int n1 = n / 2;
-  int n2 = n - n1;
-  float s1 = gt_bits_to_f32((uint32_t)(n1 + 127) << 23);
-  float s2 = gt_bits_to_f32((uint32_t)(n2 + 127) << 23);
-  return (p * s1) * s2;
+  double r = x - fn * GT_EXP64_LN2_HI;
+  r = r - fn * GT_EXP64_LN2_LO;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return p * gt_bits_to_f64((uint64_t)(n + 1023) << 52);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -236,46 +241,38 @@ GT_HD int gt_front_facing(const float* w2o, gt_f3 n, gt_f3 d_world) {
 }
 
 /* ---------------------------------------------------------------------- */
-/* Inverse covariance of computeResponse (src/GS_shader.ts:115-130).
- *   R (column-major GLSL mat3 of the quaternion (r,x,y,z)),
- *   M = invS * R  ->  M[c][r] = (1/s_r) * R[c][r]
- *   A = transpose(M) * M  ->  A[c][r] = (M[r][0]M[c][0] + M[r][1]M[c][1]) + M[r][2]M[c][2]
- * A is symmetric bit for bit; the six distinct entries are returned as
- * (a00, a01, a02, a11, a12, a22) with a_ij = A[i][j].                     */
-GT_HD void gt_inv_cov(const float* scale, const float* quat, float* a6) {
-  float r = quat[0], x = quat[1], y = quat[2], z = quat[3];
-  float R[3][3];
-  R[0][0] = 1.f - 2.f * (y * y + z * z); R[0][1] = 2.f * (x * y - r * z); R[0][2] = 2.f * (x * z + r * y);
-  R[1][0] = 2.f * (x * y + r * z); R[1][1] = 1.f - 2.f * (x * x + z * z); R[1][2] = 2.f * (y * z - r * x);
-  R[2][0] = 2.f * (x * z - r * y); R[2][1] = 2.f * (y * z + r * x); R[2][2] = 1.f - 2.f * (x * x + y * y);
-  float is[3] = {1.0f / scale[0], 1.0f / scale[1], 1.0f / scale[2]};
-  float M[3][3];
-  for (int c = 0; c < 3; ++c)
-    for (int rr = 0; rr < 3; ++rr) M[c][rr] = is[rr] * R[c][rr];
-  float A[3][3];
-  for (int c = 0; c < 3; ++c)
-    for (int rr = 0; rr < 3; ++rr) A[c][rr] = (M[rr][0] * M[c][0] + M[rr][1] * M[c][1]) + M[rr][2] * M[c][2];
-  a6[0] = A[0][0]; a6[1] = A[0][1]; a6[2] = A[0][2];
-  a6[3] = A[1][1]; a6[4] = A[1][2]; a6[5] = A[2][2];
-}
-
-/* A * v for the symmetric A of gt_inv_cov: columns left to right,
- *   (A*v)_r = (A[0][r]*v.x + A[1][r]*v.y) + A[2][r]*v.z                   */
-GT_HD gt_f3 gt_sym_mul(const float* a, gt_f3 v) {
-  return gt_f3_make((a[0] * v.x + a[1] * v.y) + a[2] * v.z,
-                    (a[1] * v.x + a[3] * v.y) + a[4] * v.z,
-                    (a[2] * v.x + a[4] * v.y) + a[5] * v.z);
-}
-
-/* Max response of the Gaussian along the ray (computeResponse, 132-135):
- *   temp = A d; p = o + (dot(mu - o, temp) / dot(d, temp)) d
- *   return exp(dot(mu - p, A (p - mu)))     (no 1/2 factor)               */
-GT_HD float gt_response(const float* a6, gt_f3 mu, gt_f3 o, gt_f3 d) {
-  gt_f3 temp = gt_sym_mul(a6, d);
-  float s = gt_dot(gt_sub(mu, o), temp) / gt_dot(d, temp);
-  gt_f3 p = gt_add(o, gt_scale(s, d));
-  gt_f3 q = gt_sym_mul(a6, gt_sub(p, mu));
-  return gt_expf(gt_dot(gt_sub(mu, p), q));
+/* Max response of the Gaussian along the ray, computeResponse
+ * (src/GS_shader.ts:111-136): exp(-min_t (o + t d - mu)^T Sigma^-1 (o + t d - mu))
+ * with Sigma^-1 = (S^-1 R)^T (S^-1 R), R the shader's column-major mat3 of the
+ * quaternion (r, x, y, z), and no 1/2 in the exponent.
+ *
+ * Evaluated in FP64 from the FP32 inputs, in the whitened frame:
+ *   a = S^-1 R (o - mu),  b = S^-1 R d,  min = |a x b|^2 / |b|^2.
+ * This is the same function as the reference's p = o + (dot(mu-o, Ad) /
+ * dot(d, Ad)) d, exp(dot(mu - p, A (p - mu))), but the literal FP32 form
+ * loses up to 3e-3 in the response on flat Gaussians (scale ratios of 500 in
+ * the reference's data/pc_short.ply), because forming A squares the
+ * condition number; FP64 keeps it within 1e-12 of the exact value.
+ * Returns the response in double; alpha = (float)(response * opacity). */
+GT_HD double gt_response64(const float* mu, const float* scale, const float* quat, gt_f3 o, gt_f3 d) {
+  const double r = quat[0], x = quat[1], y = quat[2], z = quat[3];
+  /* R[c][row] as GS_shader.ts:119-122 */
+  const double R00 = 1.0 - 2.0 * (y * y + z * z), R01 = 2.0 * (x * y - r * z), R02 = 2.0 * (x * z + r * y);
+  const double R10 = 2.0 * (x * y + r * z), R11 = 1.0 - 2.0 * (x * x + z * z), R12 = 2.0 * (y * z - r * x);
+  const double R20 = 2.0 * (x * z - r * y), R21 = 2.0 * (y * z + r * x), R22 = 1.0 - 2.0 * (x * x + y * y);
+  const double i0 = 1.0 / (double)scale[0], i1 = 1.0 / (double)scale[1], i2 = 1.0 / (double)scale[2];
+  const double u0 = (double)o.x - (double)mu[0], u1 = (double)o.y - (double)mu[1], u2 = (double)o.z - (double)mu[2];
+  const double d0 = d.x, d1 = d.y, d2 = d.z;
+  /* (invS * R) v: row k = (1/s_k) * ((R[0][k] v0 + R[1][k] v1) + R[2][k] v2) */
+  const double a0 = i0 * ((R00 * u0 + R10 * u1) + R20 * u2);
+  const double a1 = i1 * ((R01 * u0 + R11 * u1) + R21 * u2);
+  const double a2 = i2 * ((R02 * u0 + R12 * u1) + R22 * u2);
+  const double b0 = i0 * ((R00 * d0 + R10 * d1) + R20 * d2);
+  const double b1 = i1 * ((R01 * d0 + R11 * d1) + R21 * d2);
+  const double b2 = i2 * ((R02 * d0 + R12 * d1) + R22 * d2);
+  const double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+  const double e = ((c0 * c0 + c1 * c1) + c2 * c2) / ((b0 * b0 + b1 * b1) + b2 * b2);
+  return gt_exp64(-e);
 }
 
 /* Relaunch step (GS_shader.ts:197, 248): t_curr = t_k + 1e-4 in FP32. */

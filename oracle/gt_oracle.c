@@ -114,7 +114,7 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
 }
 
 /* ---------------- shading pieces ---------------- */
-float gto_expf(float x) { return gt_expf(x); }
+double gto_exp64(double x) { return gt_exp64(x); }
 
 void gto_ray_dir(const gto_params* p, uint32_t px, uint32_t py, float* d3) {
   gt_f3 d;
@@ -129,21 +129,14 @@ void gto_ray_dir(const gto_params* p, uint32_t px, uint32_t py, float* d3) {
   d3[0] = d.x; d3[1] = d.y; d3[2] = d.z;
 }
 
-static float response_of(const float* rec, gt_f3 o, gt_f3 d) {
-  float a6[6];
-  gt_inv_cov(rec + 4, rec + 8, a6);
-  gt_f3 mu = gt_f3_make(rec[0], rec[1], rec[2]);
-  if (g_mutation == 1) {
-    gt_f3 temp = gt_sym_mul(a6, d);
-    float s = gt_dot(gt_sub(mu, o), temp) / gt_dot(d, temp);
-    gt_f3 pp = gt_add(o, gt_scale(s, d));
-    gt_f3 q = gt_sym_mul(a6, gt_sub(pp, mu));
-    return gt_expf(0.5f * gt_dot(gt_sub(mu, pp), q));
-  }
-  return gt_response(a6, mu, o, d);
+/* computeResponse (GS_shader.ts:111-136) through gt_response64 */
+static double response_of(const float* rec, gt_f3 o, gt_f3 d) {
+  double r = gt_response64(rec, rec + 4, rec + 8, o, d);
+  if (g_mutation == 1) r = sqrt(r); /* exp(-e/2): a 1/2 factor in the exponent */
+  return r;
 }
 
-float gto_response(const float* rec, const float* o3, const float* d3) {
+double gto_response(const float* rec, const float* o3, const float* d3) {
   return response_of(rec, gt_f3_make(o3[0], o3[1], o3[2]), gt_f3_make(d3[0], d3[1], d3[2]));
 }
 
@@ -497,7 +490,7 @@ static void render_ray(render_job* J, uint32_t li, uint64_t* st) {
         if (id == GT_SENTINEL) { t_curr = p->t_max; break; }
         st[2]++;
         const float* rec = s->packed + (size_t)GTO_PACKED_FLOATS * id;
-        float alpha = response_of(rec, r.o, r.d) * rec[12];
+        float alpha = (float)(response_of(rec, r.o, r.d) * (double)rec[12]);
         if (p->alpha_min < alpha) {
           gt_f3 mu = gt_f3_make(rec[0], rec[1], rec[2]);
           gt_f3 dir = gt_normalize(gt_sub(mu, r.o));

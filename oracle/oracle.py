@@ -52,10 +52,10 @@ _lib.gto_set_mutation.argtypes = [C.c_int]
 _lib.gto_set_mutation.restype = None
 _lib.gto_ray_dir.argtypes = [C.POINTER(Params), C.c_uint32, C.c_uint32, _P]
 _lib.gto_ray_dir.restype = None
-_lib.gto_expf.argtypes = [C.c_float]
-_lib.gto_expf.restype = C.c_float
+_lib.gto_exp64.argtypes = [C.c_double]
+_lib.gto_exp64.restype = C.c_double
 _lib.gto_response.argtypes = [_P, _P, _P]
-_lib.gto_response.restype = C.c_float
+_lib.gto_response.restype = C.c_double
 _lib.gto_radiance.argtypes = [_P, C.c_uint32, _P, _P]
 _lib.gto_radiance.restype = None
 _lib.gto_tri_hit.argtypes = [_P, _P, C.c_float, C.c_float, C.c_int, _P]
@@ -172,8 +172,8 @@ def ray_dir(params, px, py):
     return d
 
 
-def expf(x):
-    return _lib.gto_expf(float(x))
+def exp64(x):
+    return _lib.gto_exp64(float(x))
 
 
 def response(rec, o, d):

@@ -72,16 +72,17 @@ def mt64(o, d, p0, p1, p2):
 
 
 # ---------------------------------------------------------------- pins
-def check_expf():
-    xs = np.concatenate([np.linspace(-85.9, 88.0, 120001), -np.logspace(-9, 1.9, 1500),
-                         np.logspace(-9, 1.9, 1500), [0.0, -1e-30, 1e-30]]).astype(F32)
-    got = np.array([orc.expf(float(x)) for x in xs], np.float64)
-    ref = np.exp(xs.astype(np.float64))
-    ulp = np.spacing(ref.astype(F32)).astype(np.float64)
-    err = np.abs(got - ref) / ulp
-    assert err.max() <= 2.0, f"expf error {err.max()} ulp at x={xs[err.argmax()]}"
-    assert orc.expf(-86.5) == 0.0 and orc.expf(-1e4) == 0.0 and orc.expf(float("nan")) == 0.0
-    assert math.isinf(orc.expf(89.0))
+def check_exp64():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([np.linspace(-699.0, 699.0, 60001), -np.logspace(-12, 2.5, 3000),
+                         np.logspace(-12, 2.5, 3000), rng.uniform(-40, 5, 20000), [0.0, -1e-300, 1e-300]])
+    got = np.array([orc.exp64(float(x)) for x in xs])
+    ref = np.exp(xs)
+    rel = np.abs(got - ref) / ref
+    assert rel.max() < 2e-15, f"exp64 relative error {rel.max()} at x={xs[rel.argmax()]}"
+    assert orc.exp64(-700.5) == 0.0 and orc.exp64(-1e4) == 0.0 and orc.exp64(float("nan")) == 0.0
+    assert math.isinf(orc.exp64(700.0))
+    assert orc.exp64(1.0) == math.e or abs(orc.exp64(1.0) / math.e - 1) < 3e-16
 
 
 def check_icosahedron():
@@ -386,7 +387,7 @@ def check_invariants_and_reeval():
 
 
 PINS = {
-    "expf": check_expf, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
+    "exp64": check_exp64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
     "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
     "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
     "composite": check_composite_closed_form, "relaunch": check_relaunch_window,

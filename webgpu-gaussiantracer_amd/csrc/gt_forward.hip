@@ -316,9 +316,11 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
               st.h_eval++;
               const float4* G = reinterpret_cast<const float4*>(S.geom) + 3 * (size_t)id;
               const float4 g0 = G[0], g1 = G[1], g2 = G[2];
-              const float a6[6] = {g1.x, g1.y, g1.z, g1.w, g2.x, g2.y};
+              const float mu3[3] = {g0.x, g0.y, g0.z};
+              const float sc3[3] = {g1.x, g1.y, g1.z};
+              const float q4[4] = {g2.x, g2.y, g2.z, g2.w};
               const gt_f3 mu = gt_f3_make(g0.x, g0.y, g0.z);
-              const float alpha = gt_response(a6, mu, r.o, r.d) * g0.w;
+              const float alpha = (float)(gt_response64(mu3, sc3, q4, r.o, r.d) * (double)g0.w);
               if (F.alpha_min < alpha) {
                 const gt_f3 dir = gt_normalize(gt_sub(mu, r.o));
                 const gt_f3 c = radiance(S.sh + 48 * (size_t)id, F.sh_degree, dir);
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
           kb.shift_left(1);
         }
         if (DUMP && rounds < MR) F.accept[(size_t)ray * MR + rounds] = mask;
-        if (rounds >= MR) st.overflow++;
+        if (DUMP && rounds >= MR) st.overflow++;
         rounds++;
       }
     } else if (MODE == MODE_CLOSEST) {

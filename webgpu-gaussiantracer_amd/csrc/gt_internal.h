@@ -3,11 +3,8 @@
  * HIP kernels (gt_forward.hip).
  *
  * HBM layout of a scene (all arrays 256-B aligned):
- *   geom   N  x 48 B  {mu.xyz, opacity} {a00 a01 a02 a11} {a12 a22 0 0}
- *                     a = inverse covariance of computeResponse
- *                     (src/GS_shader.ts:115-130), precomputed with the
- *                     same FP32 operations (gt_inv_cov), so the response
- *                     is bit-identical to evaluating it per hit.
+ *   geom   N  x 48 B  {mu.xyz, opacity} {scale.xyz, 0} {quat r x y z}
+ *                     (what the FP64 max-response evaluation reads)
  *   sh     N  x 192 B sh[16][3] without the vec3 padding of the 320-B record
  *   prox   N' x 64 B  {w2o c0.xyz c1.x} {c1.yz c2.xy} {c2.z c3.xyz} {gid 0 0 0}
  *                     stored in BVH leaf order, so a leaf is a contiguous run

@@ -468,7 +468,8 @@ static int build_and_upload(gt_scene* s) {
     const float* o = &h.packed[(size_t)i * 80];
     float* g = &geom[(size_t)i * 12];
     g[0] = o[0]; g[1] = o[1]; g[2] = o[2]; g[3] = o[12];
-    gt_inv_cov(o + 4, o + 8, g + 4);
+    g[4] = o[4]; g[5] = o[5]; g[6] = o[6]; g[7] = 0.0f;
+    g[8] = o[8]; g[9] = o[9]; g[10] = o[10]; g[11] = o[11];
     for (int j = 0; j < 16; ++j)
       for (int c = 0; c < 3; ++c) sh[(size_t)i * 48 + 3 * j + c] = o[16 + 4 * j + c];
   }
