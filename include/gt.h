@@ -119,6 +119,16 @@ int gt_scene_get_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids
 int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, float* packed, uint32_t* proxy_ids, float* o2w,
                     float* w2o, uint32_t* n_proxies);
 
+/* Self-check of the kernel's face pre-test on the host (no counterpart in
+ * the reference; it guards the bit-exactness of hit lists): for n pairs of
+ * a proxy world-to-object matrix (12 floats) and a world ray (o, d), writes
+ * the candidate-face mask and the mask of faces the exact triangle test and
+ * facing rule accept in (tmin, tmax).  With margin_scale 1 (the kernel's
+ * margins) exact & ~cand is 0 by design; smaller scales show how close the
+ * inputs come to the FP32 error regime. */
+int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_t n, float tmin, float tmax,
+                       float margin_scale, uint32_t* cand, uint32_t* exact);
+
 /* ---- render (traceRays) ---- */
 
 /* Per-frame outputs.  rgba: W*H float4, row-major y*W+x, A = 1

@@ -182,6 +182,107 @@ GT_HD gt_face gt_ico_face(int f) {
 }
 
 /* ---------------------------------------------------------------------- */
+/* Face pre-test.  For the ray o + t d and a directed edge a->b define the
+ * Pluecker side value
+ *   F(a->b) = (d x (a - o)) . (b - a) = d . (a x b) + (o x d) . (b - a).
+ * For a face (p0, p1, p2) of gt_ico_face, the quantities of the triangle
+ * test are beta = F(p2->p0) / dot(n, d), gamma = F(p0->p1) / dot(n, d) and
+ * 1 - beta - gamma = F(p1->p2) / dot(n, d) (the three F sum to dot(n, d)),
+ * so a FRONT face (dot(n, d) < 0) can pass only if all three F <= 0.
+ * gt_face_candidates evaluates the 30 edge values once, without division,
+ * and returns the mask of faces whose three values are <= margin_g and whose
+ * dot(n, d) <= margin_n.  With margin_g far above the FP32 error of the
+ * triangle test, every face the exact test + facing rule accepts is in the
+ * mask; the exact test then runs only on those faces. */
+#define GT_NUM_EDGES 30
+
+/* edges: 30 x 6 floats {a x b, b - a} per undirected edge (a < b);
+ * face_edges: 20 x 3 codes +-(e+1) of the directed edges (i0->i1),
+ * (i1->i2), (i2->i0) of each face (negative: edge stored as b->a). */
+static inline void gt_ico_edge_tables(float* edges, int8_t* face_edges) {
+  int ea[GT_NUM_EDGES], eb[GT_NUM_EDGES], ne = 0;
+  for (int f = 0; f < GT_NUM_FACES; ++f) {
+    for (int j = 0; j < 3; ++j) {
+      int a = gt_ico_indices[3 * f + j], b = gt_ico_indices[3 * f + (j + 1) % 3];
+      int lo = a < b ? a : b, hi = a < b ? b : a, e = -1;
+      for (int k = 0; k < ne; ++k)
+        if (ea[k] == lo && eb[k] == hi) e = k;
+      if (e < 0) { ea[ne] = lo; eb[ne] = hi; e = ne++; }
+      face_edges[3 * f + j] = (int8_t)((a == lo ? 1 : -1) * (e + 1));
+    }
+  }
+  for (int e = 0; e < GT_NUM_EDGES; ++e) {
+    gt_f3 A = gt_ico_vertex(ea[e]), B = gt_ico_vertex(eb[e]);
+    gt_f3 K = gt_cross(A, B), D = gt_sub(B, A);
+    edges[6 * e + 0] = K.x; edges[6 * e + 1] = K.y; edges[6 * e + 2] = K.z;
+    edges[6 * e + 3] = D.x; edges[6 * e + 4] = D.y; edges[6 * e + 5] = D.z;
+  }
+}
+
+/* Margins of the pre-test for an object-space ray (oo, od) and the
+ * world-to-object matrix w2o.  margin_g covers, about 7 times over, the
+ * FP32 error of the triangle test's barycentrics and of the edge values in
+ * F units, together at most ~26 ulp * |d|_1 (|o|_1 + 2) (|e| <= 1.33,
+ * |p0| <= 1.26); margin_n is the facing pre-test bound
+ * 1e-5 * max|n|_1 * sum|W2O| (the world-space facing dot and dot(n, d_obj)
+ * differ by less than 8 ulp * |n|_1 * sum|W2O|). */
+GT_HD void gt_face_margins(gt_f3 oo, gt_f3 od, const float* w2o, float* margin_g, float* margin_n) {
+  float o1 = (fabsf(oo.x) + fabsf(oo.y)) + fabsf(oo.z);
+  float d1 = (fabsf(od.x) + fabsf(od.y)) + fabsf(od.z);
+  *margin_g = 1.2e-5f * d1 * (o1 + 2.0f);
+  float ws = 0.0f;
+  for (int i = 0; i < 9; ++i) ws += fabsf(w2o[i]);
+  *margin_n = 3e-5f * ws;
+}
+
+/* The face_edges table of gt_ico_edge_tables as a literal, so that the
+ * unrolled pre-test indexes the edge values statically (checked equal to
+ * the generated table by the CPU tests). */
+#define GT_ICO_FACE_EDGES                                                  \
+  {1, -2, -3, 3, -4, -5, 5, 6, -7, 7, 8, -9, 9, 10, -1,                   \
+   4, 11, -12, 2, -13, 14, -10, -15, 16, -8, -17, 18, -6, 19, -20,        \
+   21, -22, -23, 23, -24, 25, -25, 26, -27, 27, 28, -29, 29, 30, -21,     \
+   22, -11, -14, 24, 13, -16, -26, 15, -18, -28, 17, 20, -30, -19, 12}
+GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GT_UNROLL _Pragma("unroll")
+#else
+#define GT_UNROLL
+#endif
+
+/* Slot 3f+j at which edge e is first used by face f; edge values are
+ * computed there, so at most 16 of the 30 are live at once. */
+GT_CONST_TABLE uint8_t gt_ico_edge_first[30] = {0,  1,  2,  4,  5,  7,  8,  10, 11, 13, 16, 17, 19, 20, 22,
+                                                23, 25, 26, 28, 29, 30, 31, 32, 34, 35, 37, 38, 40, 41, 43};
+
+GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, const float* edges, const float* normals, float margin_g,
+                                  float margin_n) {
+  gt_f3 w = gt_cross(oo, od);
+  float g[GT_NUM_EDGES];
+  uint32_t mask = 0;
+  GT_UNROLL
+  for (int f = 0; f < GT_NUM_FACES; ++f) {
+    const float* n = normals + 3 * f;
+    float nd = (n[0] * od.x + n[1] * od.y) + n[2] * od.z;
+    int ok = nd <= margin_n;
+    GT_UNROLL
+    for (int j = 0; j < 3; ++j) {
+      int c = gt_ico_face_edges[3 * f + j];
+      int e = (c > 0 ? c : -c) - 1;
+      if (gt_ico_edge_first[e] == 3 * f + j) {
+        const float* E = edges + 6 * e;
+        g[e] = ((od.x * E[0] + od.y * E[1]) + od.z * E[2]) + ((w.x * E[3] + w.y * E[4]) + w.z * E[5]);
+      }
+      float v = c > 0 ? g[e] : -g[e];
+      ok = ok && (v <= margin_g);
+    }
+    mask |= (uint32_t)ok << f;
+  }
+  return mask;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Ray generation (src/GS_shader.ts:277-285).  to_world is the column-major
  * camera matrix (camera.ts: columns right, up, back, position).
  *   X = -col0, Y = col1, Z = -col2

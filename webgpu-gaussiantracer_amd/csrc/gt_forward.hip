@@ -32,12 +32,40 @@
 
 namespace {
 
-/* Face constants {p0, e0, e1, n} of the 20 icosahedron faces, uploaded once
- * per device from gt_ico_face (scalar loads in the face loop). */
+/* Icosahedron tables, uploaded once per device from gt_ico_face /
+ * gt_ico_edge_tables: face constants {p0, e0, e1, n}, Pluecker edge
+ * constants {a x b, b - a}, signed edge codes per face, face normals. */
 __constant__ float c_faces[12 * GT_NUM_FACES];
+__constant__ float c_edges[6 * GT_NUM_EDGES];
+__constant__ float c_normals[3 * GT_NUM_FACES];
 
-/* max over faces of |n|_1 is 2.63; 3 bounds it */
-#define GT_FACE_N1_BOUND 3.0f
+#define GT_BLOCK 256
+#ifndef GT_MIN_WAVES
+#define GT_MIN_WAVES 4 /* waves per SIMD the register budget is sized for */
+#endif
+#define GT_LDS_STACK 32 /* stack entries per lane held in LDS; deeper ones spill to scratch */
+
+/* Traversal stack: the top GT_LDS_STACK entries live in LDS as a ring
+ * (entry i at slot i % GT_LDS_STACK, lane-interleaved so that a wave's
+ * accesses hit distinct banks); entries below spill to a private array. */
+struct Stack {
+  uint32_t* lds; /* this lane's column of the block's LDS stack */
+  int sp;
+  uint32_t spill[GT_STACK_MAX];
+  __device__ __forceinline__ void push(uint32_t x) {
+    const int slot = (sp & (GT_LDS_STACK - 1)) * GT_BLOCK;
+    if (sp >= GT_LDS_STACK) spill[sp - GT_LDS_STACK] = lds[slot];
+    lds[slot] = x;
+    ++sp;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    --sp;
+    const int slot = (sp & (GT_LDS_STACK - 1)) * GT_BLOCK;
+    const uint32_t x = lds[slot];
+    if (sp >= GT_LDS_STACK) lds[slot] = spill[sp - GT_LDS_STACK];
+    return x;
+  }
+};
 
 struct RayState {
   gt_f3 o, d;
@@ -97,13 +125,17 @@ struct Counters {
 
 enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
 
-/* Exact proxy test: object-space ray (trace.glsl:102-105), 20 faces with the
- * triangle test of intersect.glsl:73-92 and the facing rule of
- * trace.glsl:153-164.  MODE_CLOSEST keeps back faces (debugMode 1 traces with
- * no cull flag, GS_shader.ts:292-294). */
+/* Exact proxy test: object-space ray (trace.glsl:102-105), the triangle
+ * test of intersect.glsl:73-92 and the facing rule of trace.glsl:153-164.
+ * A bounding-sphere test and the Pluecker face pre-test (gt_face_candidates)
+ * select the faces that can pass; only those run the exact test, so a lane
+ * normally runs one exact test and one k-buffer insertion per proxy.
+ * MODE_CLOSEST keeps back faces (debugMode 1 traces with no cull flag,
+ * GS_shader.ts:292-294) and tests all 20 faces. */
 template <int MAXK, int MODE>
 __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, const RayState& r, float tmin,
-                                           float tmax, KBuf<MAXK>& kb, uint32_t& count, Counters& st) {
+                                           float tmax, KBuf<MAXK>& kb, uint32_t& count, Counters& st,
+                                           const float* s_faces) {
   const float4* P = reinterpret_cast<const float4*>(S.prox) + 4 * (size_t)pi;
   const float4 q0 = P[0], q1 = P[1], q2 = P[2], q3 = P[3];
   const float w[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
@@ -119,26 +151,23 @@ __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, c
     const float R = GT_ICO_CIRCUMRADIUS * 1.001f + 1e-4f * o1;
     if (gt_dot(cr, cr) > R * R * gt_dot(od, od)) return;
   }
-  float margin = 0.0f;
-  if (MODE != MODE_CLOSEST) {
-    /* facing pre-test: dot(n, d_obj) and the exact facing dot(n*W2O, d)
-     * differ by < 8 ulp * |n|_1 * sum|W2O|; faces with dot(n, d_obj) above
-     * this bound are certainly culled as back faces */
-    float ws = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) ws += fabsf(w[i]);
-    margin = 1e-5f * GT_FACE_N1_BOUND * ws;
+  uint32_t cand;
+  if (MODE == MODE_CLOSEST) {
+    cand = (1u << GT_NUM_FACES) - 1u;
+  } else {
+    float mg, mn;
+    gt_face_margins(oo, od, w, &mg, &mn);
+    cand = gt_face_candidates(oo, od, c_edges, c_normals, mg, mn);
   }
-#pragma unroll 2
-  for (int f = 0; f < GT_NUM_FACES; ++f) {
+  while (cand) {
+    const int f = __ffs(cand) - 1;
+    cand &= cand - 1u;
+    const float* F = s_faces + 12 * f;
     gt_face fc;
-    fc.p0 = gt_f3_make(c_faces[12 * f + 0], c_faces[12 * f + 1], c_faces[12 * f + 2]);
-    fc.e0 = gt_f3_make(c_faces[12 * f + 3], c_faces[12 * f + 4], c_faces[12 * f + 5]);
-    fc.e1 = gt_f3_make(c_faces[12 * f + 6], c_faces[12 * f + 7], c_faces[12 * f + 8]);
-    fc.n = gt_f3_make(c_faces[12 * f + 9], c_faces[12 * f + 10], c_faces[12 * f + 11]);
-    if (MODE != MODE_CLOSEST) {
-      if (gt_dot(fc.n, od) > margin) continue;
-    }
+    fc.p0 = gt_f3_make(F[0], F[1], F[2]);
+    fc.e0 = gt_f3_make(F[3], F[4], F[5]);
+    fc.e1 = gt_f3_make(F[6], F[7], F[8]);
+    fc.n = gt_f3_make(F[9], F[10], F[11]);
     float t;
     if (!gt_tri_hit(oo, od, tmin, tmax, fc, &t)) continue;
     if (MODE != MODE_CLOSEST && !gt_front_facing(w, fc.n, r.d)) continue;
@@ -152,99 +181,109 @@ __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, c
 }
 
 /* One traversal of the BVH for the interval (tmin, tmax): collects the
- * k smallest (t, id) hits into kb.  Children are visited near to far and
- * culled against the current k-th entry (with conservative slack). */
+ * k smallest (t, id) hits into kb.  Children are pushed far to near and
+ * culled against the current k-th entry (with conservative slack).  The
+ * loop alternates "descend until this lane holds a leaf" with "test one
+ * proxy", so the lanes of a wave run the expensive proxy test together. */
 template <int MAXK, int MODE>
 __device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, float tmax, float slack_abs,
-                      KBuf<MAXK>& kb, uint32_t& count, Counters& st) {
+                      KBuf<MAXK>& kb, uint32_t& count, Counters& st, Stack& stk, const float* s_faces) {
   if (S.root == GT_NODE_EMPTY) return;
-  uint32_t stack[GT_STACK_MAX];
-  int sp = 0;
-  stack[sp++] = S.root;
+  stk.sp = 0;
+  stk.push(S.root);
   const float lo_t = tmin * (1.0f - 2e-5f) - slack_abs;
-  while (sp > 0) {
-    const uint32_t ref = stack[--sp];
-    if (ref & GT_LEAF_BIT) {
-      const uint32_t first = ref & GT_LEAF_FIRST_MASK;
-      const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
-      for (uint32_t j = 0; j < cnt; ++j) test_proxy<MAXK, MODE>(S, first + j, r, tmin, tmax, kb, count, st);
-      continue;
-    }
-    st.nodes++;
-    const float4* N = reinterpret_cast<const float4*>(S.nodes + ref);
-    const float4 lx = N[0], hx = N[1], ly = N[2], hy = N[3], lz = N[4], hz = N[5];
-    const uint4 ch = reinterpret_cast<const uint4*>(N)[6];
-    const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
-    const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
-    const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
-    const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-    float tn[4];
-    uint32_t rr[4];
+  uint32_t pfirst = 0, pcount = 0;
+  for (;;) {
+    while (pcount == 0 && stk.sp > 0) {
+      const uint32_t ref = stk.pop();
+      if (ref & GT_LEAF_BIT) {
+        pfirst = ref & GT_LEAF_FIRST_MASK;
+        pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+        break;
+      }
+      st.nodes++;
+      const float4* N = reinterpret_cast<const float4*>(S.nodes + ref);
+      const float4 lx = N[0], hx = N[1], ly = N[2], hy = N[3], lz = N[4], hz = N[5];
+      const uint4 ch = reinterpret_cast<const uint4*>(N)[6];
+      const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
+      const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
+      const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+      const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+      float tn[4];
+      uint32_t rr[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const float ax = (clx[c] - r.o.x) * r.inv.x, bx = (chx[c] - r.o.x) * r.inv.x;
-      const float ay = (cly[c] - r.o.y) * r.inv.y, by = (chy[c] - r.o.y) * r.inv.y;
-      const float az = (clz[c] - r.o.z) * r.inv.z, bz = (chz[c] - r.o.z) * r.inv.z;
-      const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
-      const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
-      const bool hit = tnear <= tfar && cref[c] != GT_NODE_EMPTY;
-      tn[c] = hit ? tnear : INFINITY;
-      rr[c] = hit ? cref[c] : GT_NODE_EMPTY;
-    }
-    /* sort descending by tnear (misses = +inf first), push far to near */
+      for (int c = 0; c < 4; ++c) {
+        const float ax = (clx[c] - r.o.x) * r.inv.x, bx = (chx[c] - r.o.x) * r.inv.x;
+        const float ay = (cly[c] - r.o.y) * r.inv.y, by = (chy[c] - r.o.y) * r.inv.y;
+        const float az = (clz[c] - r.o.z) * r.inv.z, bz = (chz[c] - r.o.z) * r.inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+        const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+        const bool hit = tnear <= tfar && cref[c] != GT_NODE_EMPTY;
+        tn[c] = hit ? tnear : INFINITY;
+        rr[c] = hit ? cref[c] : GT_NODE_EMPTY;
+      }
+      /* sort descending by tnear (misses = +inf first), push far to near */
 #define GT_CSWAP(a, b)                                   \
   if (tn[a] < tn[b]) {                                   \
     const float t_ = tn[a]; tn[a] = tn[b]; tn[b] = t_;   \
     const uint32_t r_ = rr[a]; rr[a] = rr[b]; rr[b] = r_; \
   }
-    GT_CSWAP(0, 1) GT_CSWAP(2, 3) GT_CSWAP(0, 2) GT_CSWAP(1, 3) GT_CSWAP(1, 2)
+      GT_CSWAP(0, 1) GT_CSWAP(2, 3) GT_CSWAP(0, 2) GT_CSWAP(1, 3) GT_CSWAP(1, 2)
 #undef GT_CSWAP
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (rr[c] != GT_NODE_EMPTY) stack[sp++] = rr[c];
+      for (int c = 0; c < 4; ++c)
+        if (rr[c] != GT_NODE_EMPTY) stk.push(rr[c]);
+    }
+    if (pcount == 0) break;
+    test_proxy<MAXK, MODE>(S, pfirst, r, tmin, tmax, kb, count, st, s_faces);
+    ++pfirst;
+    --pcount;
   }
 }
 
-/* shToRadiance + computeRadiance (GS_shader.ts:138-190) */
+/* shToRadiance + computeRadiance (GS_shader.ts:138-190): the 16 real-SH
+ * basis values with the constants of GS_shader.ts:39-52, then
+ * L_c = max(0, 0.5 + sum_j b_j sh[j][c]) with the coefficients streamed
+ * 16 B at a time (pixels are checked against FP64 within 1e-4, so the FP32
+ * summation order is free; streaming keeps register pressure low). */
 __device__ __forceinline__ gt_f3 radiance(const float* __restrict__ shp, uint32_t deg, gt_f3 dir) {
   const float4* S4 = reinterpret_cast<const float4*>(shp);
-  float sh[48];
-  const int n4 = deg == 0 ? 1 : deg == 1 ? 3 : deg == 2 ? 7 : 12;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    if (i < n4) {
-      const float4 v = S4[i];
-      sh[4 * i] = v.x; sh[4 * i + 1] = v.y; sh[4 * i + 2] = v.z; sh[4 * i + 3] = v.w;
-    } else {
-      sh[4 * i] = sh[4 * i + 1] = sh[4 * i + 2] = sh[4 * i + 3] = 0.0f;
-    }
-  }
   const float x = dir.x, y = dir.y, z = dir.z;
   const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
-  float out[3];
+  float b[16];
+  b[0] = 0.28209479177387814f;
+  b[1] = -0.4886025119029199f * y;
+  b[2] = 0.4886025119029199f * z;
+  b[3] = -0.4886025119029199f * x;
+  b[4] = 1.0925484305920792f * xy;
+  b[5] = -1.0925484305920792f * yz;
+  b[6] = 0.31539156525252005f * ((2.f * zz - xx) - yy);
+  b[7] = -1.0925484305920792f * xz;
+  b[8] = 0.5462742152960396f * (xx - yy);
+  b[9] = -0.5900435899266435f * y * (3.0f * xx - yy);
+  b[10] = 2.890611442640554f * xy * z;
+  b[11] = -0.4570457994644658f * y * ((4.0f * zz - xx) - yy);
+  b[12] = 0.3731763325901154f * z * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+  b[13] = -0.4570457994644658f * x * ((4.0f * zz - xx) - yy);
+  b[14] = 1.445305721320277f * z * (xx - yy);
+  b[15] = -0.5900435899266435f * x * (xx - 3.0f * yy);
+  const int nb = deg == 0 ? 1 : deg == 1 ? 4 : deg == 2 ? 9 : 16;
+  float acc[3] = {0.5f, 0.5f, 0.5f};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#define SH(i) sh[3 * (i) + c]
-    float L = 0.5f + 0.28209479177387814f * SH(0);
-    if (deg >= 1) L += 0.4886025119029199f * ((-y * SH(1) + z * SH(2)) - x * SH(3));
-    if (deg >= 2)
-      L += (((1.0925484305920792f * xy * SH(4) + -1.0925484305920792f * yz * SH(5)) +
-             0.31539156525252005f * ((2.f * zz - xx) - yy) * SH(6)) +
-            -1.0925484305920792f * xz * SH(7)) +
-           0.5462742152960396f * (xx - yy) * SH(8);
-    if (deg >= 3)
-      L += (((((-0.5900435899266435f * y * (3.0f * xx - yy) * SH(9) + 2.890611442640554f * xy * z * SH(10)) +
-               -0.4570457994644658f * y * ((4.0f * zz - xx) - yy) * SH(11)) +
-              0.3731763325901154f * z * ((2.0f * zz - 3.0f * xx) - 3.0f * yy) * SH(12)) +
-             -0.4570457994644658f * x * ((4.0f * zz - xx) - yy) * SH(13)) +
-            1.445305721320277f * z * (xx - yy) * SH(14)) +
-           -0.5900435899266435f * x * (xx - 3.0f * yy) * SH(15);
-#undef SH
-    out[c] = L > 0.0f ? L : 0.0f;
+  for (int i = 0; i < 12; ++i) {
+    if (3 * nb > 4 * i) {
+      const float4 v = S4[i];
+      const float q[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int idx = 4 * i + k, j = idx / 3, c = idx % 3;
+        if (j < nb) acc[c] += b[j] * q[k];
+      }
+    }
   }
-  return gt_f3_make(out[0], out[1], out[2]);
+  return gt_f3_make(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f));
 }
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
@@ -260,7 +299,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 template <int MAXK, int MODE, bool DUMP, bool STATS>
-__global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_dev_frame F) {
+__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F) {
+  __shared__ uint32_t s_stack[GT_LDS_STACK * GT_BLOCK];
+  __shared__ float s_faces[12 * GT_NUM_FACES];
+  for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
+  __syncthreads();
+  Stack stk;
+  stk.lds = s_stack + threadIdx.x;
+  stk.sp = 0;
   const uint32_t tiles_x = (F.width + 7u) >> 3;
   const uint32_t tiles_y = (F.height + 7u) >> 3;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -295,7 +341,7 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
       while (F.T_min < T && t_curr < F.t_max) {
         if (F.early_stop == step++) break;
         kb.reset(off, F.t_max);
-        trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st);
+        trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
         t_curr = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
         /* bring the active slots to the front (uniform shifts) */
         if (off & 1u) kb.shift_left(1);
@@ -341,7 +387,7 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
     } else if (MODE == MODE_CLOSEST) {
       /* debugMode 1 (GS_shader.ts:289-296, 312-321) */
       kb.reset(0, F.t_max);
-      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st);
+      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t id = kb.id[MAXK - 1];
       if (id != GT_SENTINEL) L = radiance(S.sh + 48 * (size_t)id, F.sh_degree, r.d);
       rounds = 1;
@@ -352,7 +398,7 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
     } else {
       /* debugMode 2 (GS_shader.ts:201-228): front-face hits in (t_min, t_max) */
       kb.reset(0, F.t_max);
-      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st);
+      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t c = count;
       float cr, cg, cb;
       if (c == 0) { cr = 0.f; cg = 0.f; cb = 0.f; }
@@ -398,7 +444,7 @@ __global__ __launch_bounds__(256) void k_forward(const gt_dev_scene S, const gt_
 template <int MAXK, int MODE>
 hipError_t launch4(const gt_dev_scene& S, const gt_dev_frame& F, hipStream_t st, bool dump, bool stats) {
   const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
-  const dim3 grid((tiles + 3u) / 4u), block(256);
+  const dim3 grid((tiles + 3u) / 4u), block(GT_BLOCK);
   if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true>), grid, block, 0, st, S, F);
   else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false>), grid, block, 0, st, S, F);
   else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true>), grid, block, 0, st, S, F);
@@ -422,7 +468,16 @@ hipError_t ensure_faces() {
                          fc.e1.x, fc.e1.y, fc.e1.z, fc.n.x,  fc.n.y,  fc.n.z};
     for (int j = 0; j < 12; ++j) f[12 * i + j] = v[j];
   }
+  float edges[6 * GT_NUM_EDGES], normals[3 * GT_NUM_FACES];
+  int8_t face_edges[3 * GT_NUM_FACES];
+  gt_ico_edge_tables(edges, face_edges);
+  for (int i = 0; i < GT_NUM_FACES; ++i)
+    for (int j = 0; j < 3; ++j) normals[3 * i + j] = f[12 * i + 9 + j];
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_faces), f, sizeof(f), 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_edges), edges, sizeof(edges), 0, hipMemcpyHostToDevice);
+  for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
+    if (face_edges[i] != gt_ico_face_edges[i]) return hipErrorInvalidValue; /* literal table out of date */
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_normals), normals, sizeof(normals), 0, hipMemcpyHostToDevice);
   if (e == hipSuccess && dev >= 0 && dev < 64) g_faces_ready[dev] = true;
   return e;
 }

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgt.so")
+LIB_PATH = os.environ.get("GTRACE_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libgt.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libgt.so not built at {LIB_PATH}; run __graft_entry__.build()")
@@ -87,6 +87,7 @@ _lib.gt_scene_destroy.restype = None
 _lib.gt_scene_get_info.argtypes = [_P, C.POINTER(SceneInfo)]
 _lib.gt_scene_get_host_data.argtypes = [_P, _P, _P, _P, _P]
 _lib.gt_host_prepare.argtypes = [_P, C.c_uint32, C.c_double, _P, _P, _P, _P, C.POINTER(C.c_uint32)]
+_lib.gt_debug_face_test.argtypes = [_P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, _P, _P]
 _lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
 _lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
 _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
@@ -97,7 +98,7 @@ _lib.gt_free.restype = None
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_host_prepare", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
+    "gt_render", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
 
 
@@ -188,6 +189,19 @@ def read_ply(path):
     finally:
         _lib.gt_free(C.cast(p, C.c_void_p))
     return raw, deg.value
+
+
+def debug_face_test(w2o, o, d, tmin=1e-3, tmax=1e5, margin_scale=1.0):
+    """Candidate and exact face masks of (proxy, ray) pairs (kernel pre-test self-check)."""
+    w2o = np.ascontiguousarray(w2o, np.float32).reshape(-1, 12)
+    o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    n = w2o.shape[0]
+    cand = np.zeros(n, np.uint32)
+    exact = np.zeros(n, np.uint32)
+    _check(_lib.gt_debug_face_test(_ptr(w2o), _ptr(o), _ptr(d), n, tmin, tmax, margin_scale, _ptr(cand),
+                                   _ptr(exact)))
+    return cand, exact
 
 
 def host_prepare(raw, alpha_min):
