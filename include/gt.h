@@ -129,6 +129,24 @@ int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, float* packe
 int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_t n, float tmin, float tmax,
                        float margin_scale, uint32_t* cand, uint32_t* exact);
 
+/* Launch tuning of a scene (no counterpart in the reference; chosen per
+ * scene by the autotuner).  Zero fields mean "library default".  None of
+ * them changes a result: hit lists and pixels are identical for all values.
+ *   path          0 default, 1 one 8x8-tile block per wave, 2 persistent
+ *                 kernel with a global ray queue
+ *   refill_min    persistent: idle lanes of a wave that trigger a refill (1..64)
+ *   comp_min      persistent: lanes waiting to composite that trigger a
+ *                 composite pass (1..64)
+ *   blocks_per_cu persistent: resident 256-thread blocks per CU (1..8) */
+typedef struct gt_tuning {
+  uint32_t path;
+  uint32_t refill_min;
+  uint32_t comp_min;
+  uint32_t blocks_per_cu;
+} gt_tuning;
+int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t);
+int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t);
+
 /* ---- render (traceRays) ---- */
 
 /* Per-frame outputs.  rgba: W*H float4, row-major y*W+x, A = 1

@@ -441,6 +441,267 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
   }
 }
 
+/* ------------------------------------------------------------------------
+ * Persistent renderer.  A grid sized to the machine; every lane runs a
+ * small state machine over rays taken from a global work counter:
+ *   TRAV  descend the BVH until it holds a leaf, then test one proxy
+ *   COMP  round traversed; waits, then composites its k entries
+ *   IDLE  takes the next ray (idle lanes of a wave take consecutive ray ids
+ *         with one atomic, so a wave stays on neighbouring pixels)
+ * Each iteration the wave votes which phase to run: a refill once
+ * REFILL_MIN lanes are idle, a composite pass once COMP_MIN lanes wait (or
+ * nothing traverses), else a traversal step.  Lanes never wait for the
+ * slowest ray of their wave: a lane whose ray ends starts another.  Ray id g -> tile g/64 (8x8 pixels), pixel g%64;
+ * tiles of all cameras of a batch are numbered consecutively.
+ * Per-ray semantics are those of k_forward (GS_shader.ts:192-273).
+ * --------------------------------------------------------------------- */
+enum { PH_IDLE = 0, PH_TRAV = 1, PH_COMP = 2 };
+
+
+template <int MAXK>
+__device__ __forceinline__ void visit_node(const gt_dev_scene& S, uint32_t ref, const RayState& r, float lo_t,
+                                           float hi_t, Stack& stk) {
+  const float4* N = reinterpret_cast<const float4*>(S.nodes + ref);
+  const float4 lx = N[0], hx = N[1], ly = N[2], hy = N[3], lz = N[4], hz = N[5];
+  const uint4 ch = reinterpret_cast<const uint4*>(N)[6];
+  const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
+  const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
+  const float clz[4] = {lz.x, lz.y, lz.z, lz.w}, chz[4] = {hz.x, hz.y, hz.z, hz.w};
+  const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+  float tn[4];
+  uint32_t rr[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float ax = (clx[c] - r.o.x) * r.inv.x, bx = (chx[c] - r.o.x) * r.inv.x;
+    const float ay = (cly[c] - r.o.y) * r.inv.y, by = (chy[c] - r.o.y) * r.inv.y;
+    const float az = (clz[c] - r.o.z) * r.inv.z, bz = (chz[c] - r.o.z) * r.inv.z;
+    const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+    const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+    const bool hit = tnear <= tfar && cref[c] != GT_NODE_EMPTY;
+    tn[c] = hit ? tnear : INFINITY;
+    rr[c] = hit ? cref[c] : GT_NODE_EMPTY;
+  }
+#define GT_CSWAP(a, b)                                   \
+  if (tn[a] < tn[b]) {                                   \
+    const float t_ = tn[a]; tn[a] = tn[b]; tn[b] = t_;   \
+    const uint32_t r_ = rr[a]; rr[a] = rr[b]; rr[b] = r_; \
+  }
+  GT_CSWAP(0, 1) GT_CSWAP(2, 3) GT_CSWAP(0, 2) GT_CSWAP(1, 3) GT_CSWAP(1, 2)
+#undef GT_CSWAP
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (rr[c] != GT_NODE_EMPTY) stk.push(rr[c]);
+}
+
+__device__ __forceinline__ uint32_t find_camera(const gt_dev_frame* cams, uint32_t n, uint32_t tile) {
+  uint32_t lo = 0, hi = n; /* last camera with tile_base <= tile */
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cams[mid].tile_base <= tile) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int MAXK, bool BATCH, bool DUMP, bool STATS>
+__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_scene S, const gt_dev_frame cam0,
+                                                                   const gt_dev_frame* __restrict__ cams,
+                                                                   uint32_t n_cams, uint32_t total_rays,
+                                                                   unsigned int* work,
+                                                                   unsigned long long* stats,
+                                                                   uint32_t REFILL_MIN, uint32_t COMP_MIN) {
+  __shared__ uint32_t s_stack[GT_LDS_STACK * GT_BLOCK];
+  __shared__ float s_faces[12 * GT_NUM_FACES];
+  for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
+  __syncthreads();
+#define CAM(field) (BATCH ? cams[cam].field : cam0.field)
+  Stack stk;
+  stk.lds = s_stack + threadIdx.x;
+  stk.sp = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t k = cam0.k;
+  const uint32_t off = (uint32_t)MAXK - k;
+  Counters st;
+  uint32_t n_rays = 0;
+  uint32_t phase = PH_IDLE, cam = 0, pix = 0;
+  RayState r;
+  r.o = gt_f3_make(0.f, 0.f, 0.f); r.d = r.o; r.inv = r.o;
+  gt_f3 L = r.o;
+  float T = 1.0f, t_curr = 0.0f, t_next = 0.0f, t_max = cam0.t_max, slack = cam0.slack_abs;
+  uint32_t step = 0, rounds = 0, pfirst = 0, pcount = 0;
+  bool exhausted = false;
+  KBuf<MAXK> kb;
+  kb.reset(off, t_max);
+  uint32_t dummy_count = 0;
+
+  /* starts the next round of the current ray or finishes it (GS_shader.ts:231-234) */
+  auto next_round_or_finish = [&]() {
+    bool go = CAM(T_min) < T && t_curr < t_max;
+    if (go && CAM(early_stop) == step++) go = false;
+    if (go) {
+      kb.reset(off, t_max);
+      stk.sp = 0;
+      if (S.root != GT_NODE_EMPTY) stk.push(S.root);
+      pcount = 0;
+      phase = PH_TRAV;
+      return;
+    }
+    float4* out = reinterpret_cast<float4*>(CAM(rgba));
+    out[pix] = make_float4(L.x, L.y, L.z, 1.0f);
+    uint32_t* nr = CAM(n_rounds);
+    if (nr) nr[pix] = rounds;
+    float* To = CAM(T_out);
+    if (To) To[pix] = T;
+    if (DUMP) {
+      const size_t MR = CAM(max_rounds);
+      uint32_t* H = CAM(hit_ids);
+      uint32_t* A = CAM(accept);
+      for (size_t rd = rounds; rd < MR; ++rd) {
+        for (uint32_t jj = 0; jj < k; ++jj) H[((size_t)pix * MR + rd) * k + jj] = GT_SENTINEL;
+        A[(size_t)pix * MR + rd] = 0;
+      }
+    }
+    st.rounds += rounds;
+    phase = PH_IDLE;
+  };
+
+  for (;;) {
+    const uint64_t mI = __ballot(phase == PH_IDLE);
+    const uint64_t mT = __ballot(phase == PH_TRAV);
+    const uint64_t mC = __ballot(phase == PH_COMP);
+    /* 1. idle lanes take new rays, in batches */
+    if (!exhausted && mI && ((uint32_t)__popcll(mI) >= REFILL_MIN || (mT | mC) == 0)) {
+      const uint32_t n = (uint32_t)__popcll(mI);
+      const int leader = __ffsll((long long)mI) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(work, n);
+      base = __shfl(base, leader, 64);
+      if (base + n >= total_rays) exhausted = true;
+      if (phase == PH_IDLE) {
+        const uint32_t g = base + (uint32_t)__popcll(mI & lt_mask);
+        if (g < total_rays) {
+          const uint32_t tile = g >> 6, sub = g & 63u;
+          if (BATCH) cam = find_camera(cams, n_cams, tile);
+          const uint32_t lt = tile - CAM(tile_base), tx = CAM(tiles_x);
+          const uint32_t px = (lt % tx) * 8u + (sub & 7u), py = (lt / tx) * 8u + (sub >> 3);
+          const uint32_t W = CAM(width), H = CAM(height);
+          if (px < W && py < H) {
+            pix = py * W + px;
+            r.o = gt_f3_make(CAM(to_world[12]), CAM(to_world[13]), CAM(to_world[14]));
+            float tw[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) tw[i] = CAM(to_world[i]);
+            r.d = gt_ray_dir(tw, CAM(ax), CAM(ay), px, py, W, H);
+            const float dx = fabsf(r.d.x) < 1e-20f ? copysignf(1e-20f, r.d.x) : r.d.x;
+            const float dy = fabsf(r.d.y) < 1e-20f ? copysignf(1e-20f, r.d.y) : r.d.y;
+            const float dz = fabsf(r.d.z) < 1e-20f ? copysignf(1e-20f, r.d.z) : r.d.z;
+            r.inv = gt_f3_make(1.0f / dx, 1.0f / dy, 1.0f / dz);
+            t_max = CAM(t_max);
+            slack = CAM(slack_abs);
+            L = gt_f3_make(0.f, 0.f, 0.f);
+            T = 1.0f;
+            t_curr = CAM(t_min);
+            step = 0;
+            rounds = 0;
+            n_rays++;
+            next_round_or_finish();
+          }
+        }
+      }
+      continue;
+    }
+    if ((mT | mC) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+
+    /* 2. lanes whose round traversal ended composite the whole round together
+     *    (GS_shader.ts:248-269) once enough of them wait or nothing else runs */
+    if (mC && ((uint32_t)__popcll(mC) >= COMP_MIN || mT == 0)) {
+      if (phase == PH_COMP) {
+        bool stop = false;
+        uint32_t mask = 0;
+        const size_t MR = DUMP ? CAM(max_rounds) : 0;
+        for (uint32_t jj = 0; jj < k; ++jj) {
+          const uint32_t id = kb.id[0];
+          if (DUMP && rounds < MR) CAM(hit_ids)[((size_t)pix * MR + rounds) * k + jj] = id;
+          if (!stop) {
+            if (id == GT_SENTINEL) {
+              stop = true;
+            } else {
+              st.h_eval++;
+              const float4* G = reinterpret_cast<const float4*>(S.geom) + 3 * (size_t)id;
+              const float4 g0 = G[0], g1 = G[1], g2 = G[2];
+              const float mu3[3] = {g0.x, g0.y, g0.z};
+              const float sc3[3] = {g1.x, g1.y, g1.z};
+              const float q4[4] = {g2.x, g2.y, g2.z, g2.w};
+              const float alpha = (float)(gt_response64(mu3, sc3, q4, r.o, r.d) * (double)g0.w);
+              if (CAM(alpha_min) < alpha) {
+                const gt_f3 dir = gt_normalize(gt_sub(gt_f3_make(g0.x, g0.y, g0.z), r.o));
+                const gt_f3 c = radiance(S.sh + 48 * (size_t)id, CAM(sh_degree), dir);
+                const float Ta = T * alpha;
+                L = gt_add(L, gt_scale(Ta, c));
+                T *= 1.0f - alpha;
+                mask |= 1u << jj;
+                st.h_comp++;
+              }
+            }
+          }
+          kb.shift_left(1);
+        }
+        if (DUMP) {
+          if (rounds < MR) CAM(accept)[(size_t)pix * MR + rounds] = mask;
+          else st.overflow++;
+        }
+        rounds++;
+        t_curr = stop ? t_max : t_next;
+        next_round_or_finish();
+      }
+      continue;
+    }
+
+    /* 3. traversal: descend to a leaf, then test one proxy */
+    if (phase == PH_TRAV) {
+      const float lo_t = t_curr * (1.0f - 2e-5f) - slack;
+      while (pcount == 0 && stk.sp > 0) {
+        const uint32_t ref = stk.pop();
+        if (ref & GT_LEAF_BIT) {
+          pfirst = ref & GT_LEAF_FIRST_MASK;
+          pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+          break;
+        }
+        st.nodes++;
+        visit_node<MAXK>(S, ref, r, lo_t, kb.t[MAXK - 1] * (1.0f + 2e-5f) + slack, stk);
+      }
+      if (pcount > 0) {
+        test_proxy<MAXK, MODE_RENDER>(S, pfirst, r, t_curr, t_max, kb, dummy_count, st, s_faces);
+        ++pfirst;
+        --pcount;
+      } else {
+        /* round traversal done: t_curr = t_k + 1e-4 (GS_shader.ts:248) */
+        t_next = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
+        if (off & 1u) kb.shift_left(1);
+        if (off & 2u) kb.shift_left(2);
+        if (off & 4u) kb.shift_left(4);
+        if (off & 8u) kb.shift_left(8);
+        if (off & 16u) kb.shift_left(16);
+        phase = PH_COMP;
+      }
+    }
+  }
+#undef CAM
+  if (STATS) {
+    const uint32_t v[8] = {n_rays, st.rounds, st.h_eval, st.h_comp, st.proxies, st.hits, st.nodes, st.overflow};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t sum = wave_sum(v[i]);
+      if (lane == 0 && sum) atomicAdd(stats + i, (unsigned long long)sum);
+    }
+  }
+}
+
+__global__ void k_reset_counter(unsigned int* work) { *work = 0u; }
+
 template <int MAXK, int MODE>
 hipError_t launch4(const gt_dev_scene& S, const gt_dev_frame& F, hipStream_t st, bool dump, bool stats) {
   const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
@@ -450,6 +711,29 @@ hipError_t launch4(const gt_dev_scene& S, const gt_dev_frame& F, hipStream_t st,
   else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true>), grid, block, 0, st, S, F);
   else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false>), grid, block, 0, st, S, F);
   return hipGetLastError();
+}
+
+template <int MAXK, bool BATCH>
+hipError_t launch_persistent(const gt_dev_scene& S, const gt_dev_frame& c0, const gt_dev_frame* cams, uint32_t n,
+                             uint32_t total_rays, unsigned int* work, unsigned long long* stats, hipStream_t st,
+                             bool dump, int grid, uint32_t refill, uint32_t comp) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, work);
+  const dim3 g(grid), b(GT_BLOCK);
+#define GT_ARGS S, c0, cams, n, total_rays, work, stats, refill, comp
+  if (dump && stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, true>), g, b, 0, st, GT_ARGS);
+  else if (dump) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, false>), g, b, 0, st, GT_ARGS);
+  else if (stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, false, true>), g, b, 0, st, GT_ARGS);
+  else hipLaunchKernelGGL((k_render<MAXK, BATCH, false, false>), g, b, 0, st, GT_ARGS);
+#undef GT_ARGS
+  return hipGetLastError();
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return cus;
 }
 
 std::mutex g_init_mu;
@@ -512,6 +796,46 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, v
   }
   if (e != hipSuccess) {
     gt_set_error("forward launch failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  return GT_OK;
+}
+
+extern "C" int gt_launch_persistent(const gt_dev_scene* S, const gt_dev_frame* cam0, const gt_dev_frame* cams_dev,
+                                    uint32_t n_cams, uint32_t total_tiles, unsigned int* work,
+                                    unsigned long long* stats, const gt_tuning* tune, void* stream) {
+  hipError_t e = ensure_faces();
+  if (e != hipSuccess) {
+    gt_set_error("face table upload failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  if ((cam0->hit_ids == nullptr) != (cam0->accept == nullptr)) {
+    gt_set_error("hit_ids and accept must be requested together");
+    return GT_E_INVALID;
+  }
+  if ((uint64_t)total_tiles * 64ull > 0xffffff00ull) {
+    gt_set_error("%u tiles exceed the 2^32 ray ids of one launch", total_tiles);
+    return GT_E_LIMIT;
+  }
+  const bool dump = cam0->hit_ids != nullptr && cam0->max_rounds > 0;
+  const bool batch = n_cams > 1;
+  const uint32_t rays = total_tiles * 64u;
+  const uint32_t k = cam0->k;
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_MIN_WAVES;
+  const int grid = device_cus() * (int)bpc;
+  const uint32_t refill = tune && tune->refill_min ? tune->refill_min : 16u;
+  const uint32_t comp = tune && tune->comp_min ? tune->comp_min : 16u;
+#define GT_DISPATCH(MK)                                                                                              \
+  (batch ? launch_persistent<MK, true>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill, comp) \
+         : launch_persistent<MK, false>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill, comp))
+  if (k <= 4) e = GT_DISPATCH(4);
+  else if (k <= 8) e = GT_DISPATCH(8);
+  else if (k <= 16) e = GT_DISPATCH(16);
+  else e = GT_DISPATCH(32);
+#undef GT_DISPATCH
+  if (e != hipSuccess) {
+    gt_set_error("render launch failed: %s", hipGetErrorString(e));
     return GT_E_HIP;
   }
   return GT_OK;

@@ -16,6 +16,8 @@
 
 #include <stdint.h>
 
+#include "../../include/gt.h"
+
 #define GT_NODE_EMPTY 0xffffffffu
 #define GT_LEAF_BIT 0x80000000u
 #define GT_LEAF_MAX 8u                /* proxies per leaf */
@@ -55,6 +57,8 @@ typedef struct gt_dev_frame {
   uint32_t width, height;
   float slack_abs; /* box-test slack, |camera origin| dependent */
   uint32_t max_rounds;
+  uint32_t tiles_x;   /* 8x8 pixel tiles per row */
+  uint32_t tile_base; /* first global tile index of this camera in a batch */
   float* rgba;
   uint32_t* hit_ids;
   uint32_t* accept;
@@ -68,6 +72,13 @@ extern "C" {
 #endif
 /* implemented in gt_forward.hip */
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, void* stream);
+/* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
+ * array of n_cams frames (tile_base set, ascending); cam0: host copy of
+ * frame 0, used directly when n_cams == 1.  work: one device counter the
+ * launch resets and consumes; stats: optional device counters. */
+int gt_launch_persistent(const gt_dev_scene* s, const gt_dev_frame* cam0, const gt_dev_frame* cams_dev,
+                         uint32_t n_cams, uint32_t total_tiles, unsigned int* work, unsigned long long* stats,
+                         const gt_tuning* tune, void* stream);
 void gt_set_error(const char* fmt, ...);
 #ifdef __cplusplus
 }

@@ -53,6 +53,8 @@ struct gt_scene {
   // device
   int device = 0;
   DevBuf geom, sh, prox, nodes, faces;
+  DevBuf ctl; /* work counter of the persistent renderer */
+  gt_tuning tune{};
   gt_dev_scene dev{};
   uint32_t n_nodes = 0, max_stack = 0, depth = 0;
   double build_ms = 0.0;
@@ -481,6 +483,10 @@ static int build_and_upload(gt_scene* s) {
   if ((rc = upload(s->prox, prox.data(), prox.size())) != GT_OK) return rc;
   if ((rc = upload(s->nodes, nodes.data(), nodes.size())) != GT_OK) return rc;
   if ((rc = upload(s->faces, faces, 12 * GT_NUM_FACES)) != GT_OK) return rc;
+  if (!s->ctl.p) {
+    const uint32_t zero[64] = {0};
+    if ((rc = upload(s->ctl, zero, 64)) != GT_OK) return rc;
+  }
   s->dev.geom = (const float*)s->geom.p;
   s->dev.sh = (const float*)s->sh.p;
   s->dev.prox = (const float*)s->prox.p;
@@ -638,6 +644,34 @@ void gt_fill_dev_frame(const gt_uniforms* u, uint32_t width, uint32_t height, gt
   f->height = height;
   const float om = std::max(std::fabs(u->to_world[12]), std::max(std::fabs(u->to_world[13]), std::fabs(u->to_world[14])));
   f->slack_abs = 2e-5f * (om + 1.0f);
+  f->tiles_x = (width + 7u) / 8u;
+  f->tile_base = 0;
+}
+
+/* The per-tile kernel by default (faster on a single camera: the rays of a
+ * wave stay coherent), the persistent kernel with tuning path 2. */
+static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
+  if (f->debug_mode != 0 || s->tune.path != 2) return gt_launch_forward(&s->dev, f, stream);
+  const uint32_t tiles = f->tiles_x * ((f->height + 7u) / 8u);
+  return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
+                              &s->tune, stream);
+}
+
+extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
+  if (!s || !t) { gt_set_error("gt_scene_set_tuning: NULL argument"); return GT_E_INVALID; }
+  if (t->path > 2 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8) {
+    gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u)", t->path,
+                 t->refill_min, t->comp_min, t->blocks_per_cu);
+    return GT_E_INVALID;
+  }
+  s->tune = *t;
+  return GT_OK;
+}
+
+extern "C" int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t) {
+  if (!s || !t) { gt_set_error("gt_scene_get_tuning: NULL argument"); return GT_E_INVALID; }
+  *t = s->tune;
+  return GT_OK;
 }
 
 const gt_dev_scene* gt_scene_dev(const gt_scene* s) { return &s->dev; }
@@ -663,7 +697,7 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     f.T_out = out->T;
     f.stats = (unsigned long long*)out->stats;
     f.max_rounds = (uint32_t)MR;
-    return gt_launch_forward(&s->dev, &f, stream);
+    return launch_frame(s, &f, stream);
   }
   // host buffers: device temporaries, synchronous
   struct Tmp {
@@ -703,7 +737,7 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     f.stats = (unsigned long long*)t_stats.p;
   }
   f.max_rounds = (uint32_t)MR;
-  if ((rc = gt_launch_forward(&s->dev, &f, nullptr)) != GT_OK) return rc;
+  if ((rc = launch_frame(s, &f, nullptr)) != GT_OK) return rc;
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(out->rgba, f.rgba, R * 16, hipMemcpyDeviceToHost));
   if (out->hit_ids) HIP_TRY(hipMemcpy(out->hit_ids, f.hit_ids, R * MR * u->k * 4, hipMemcpyDeviceToHost));

@@ -63,6 +63,11 @@ class SceneInfo(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("path", C.c_uint32), ("refill_min", C.c_uint32), ("comp_min", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32)]
+
+
 class GenParams(C.Structure):
     _fields_ = [
         ("n", C.c_uint32), ("n_clusters", C.c_uint32), ("box_half", C.c_float),
@@ -88,6 +93,8 @@ _lib.gt_scene_get_info.argtypes = [_P, C.POINTER(SceneInfo)]
 _lib.gt_scene_get_host_data.argtypes = [_P, _P, _P, _P, _P]
 _lib.gt_host_prepare.argtypes = [_P, C.c_uint32, C.c_double, _P, _P, _P, _P, C.POINTER(C.c_uint32)]
 _lib.gt_debug_face_test.argtypes = [_P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, _P, _P]
+_lib.gt_scene_set_tuning.argtypes = [_P, C.POINTER(Tuning)]
+_lib.gt_scene_get_tuning.argtypes = [_P, C.POINTER(Tuning)]
 _lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
 _lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
 _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
@@ -98,7 +105,7 @@ _lib.gt_free.restype = None
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
+    "gt_render", "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
 
 
@@ -251,6 +258,15 @@ class Scene:
 
     def set_alpha_min(self, alpha_min):
         _check(_lib.gt_scene_set_alpha_min(self._h, float(alpha_min)))
+
+    def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0):
+        """Launch tuning (results do not depend on it); 0 = library default."""
+        _check(_lib.gt_scene_set_tuning(self._h, C.byref(Tuning(path, refill_min, comp_min, blocks_per_cu))))
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(_lib.gt_scene_get_tuning(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
     def info(self):
         i = SceneInfo()
