@@ -137,12 +137,17 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
  *   refill_min    persistent: idle lanes of a wave that trigger a refill (1..64)
  *   comp_min      persistent: lanes waiting to composite that trigger a
  *                 composite pass (1..64)
- *   blocks_per_cu persistent: resident 256-thread blocks per CU (1..8) */
+ *   blocks_per_cu persistent: resident 256-thread blocks per CU (1..8)
+ *   tile_w        ray tile of a wave: tile_w x (64 / tile_w) pixels, tile_w in {4, 8, 16, 32}
+ *   stack_lds     traversal stack entries per ray kept in LDS (8, 16, 32 or 64);
+ *                 deeper entries spill to scratch memory */
 typedef struct gt_tuning {
   uint32_t path;
   uint32_t refill_min;
   uint32_t comp_min;
   uint32_t blocks_per_cu;
+  uint32_t tile_w;
+  uint32_t stack_lds;
 } gt_tuning;
 int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t);
 int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t);
@@ -175,6 +180,40 @@ typedef struct gt_frame {
  * limits: 1..16384 each, 1 <= k <= 32, t_min >= 0, sh_degree_max <= 3. */
 int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
               int buf_flags, void* hip_stream);
+
+/* ---- batched cameras (no counterpart in the reference, which renders one
+ * camera per traceRays dispatch, pass_encoder.ts:35-65) ---- */
+typedef struct gt_batch gt_batch;
+
+/* n cameras of uneven resolution sharing k and debugMode 0.  outs[i] holds
+ * DEVICE pointers (rgba required; hit_ids/accept with max_rounds for all
+ * cameras or none; outs[0].stats, if set, accumulates the batch's
+ * counters).  The camera table is copied to HBM here. */
+int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, const uint32_t* widths, const uint32_t* heights,
+                    const gt_frame* outs, gt_batch** out);
+/* Renders every camera with one launch on stream (asynchronous).  With
+ * use_graph != 0 the launches are captured into a hipGraph on first use and
+ * the graph is replayed afterwards (recaptured if the scene's tuning
+ * changes).  Each camera's results equal gt_render on that camera. */
+int gt_batch_render(gt_batch* b, int use_graph, void* hip_stream);
+int gt_batch_rays(const gt_batch* b, uint64_t* rays);
+void gt_batch_destroy(gt_batch* b);
+
+/* ---- autotuner (no counterpart in the reference) ----
+ * Times the result-neutral gt_tuning candidates on the caller's workload
+ * (iters timed renders each, after one warm-up) and stores the fastest in
+ * the scene.  Device buffers; synchronises the stream. */
+typedef struct gt_autotune_result {
+  gt_tuning best;
+  double best_ms;   /* per render, best candidate */
+  double first_ms;  /* per render, first candidate (tile kernel, 8x8, 16-entry LDS stack) */
+  uint32_t n_tried;
+  uint32_t _pad;
+} gt_autotune_result;
+int gt_autotune_frame(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, float* rgba_dev,
+                      uint32_t iters, void* hip_stream, gt_autotune_result* out);
+int gt_autotune_batch(gt_scene* s, gt_batch* b, int use_graph, uint32_t iters, void* hip_stream,
+                      gt_autotune_result* out);
 
 /* ---- synthetic scenes (seeded SplitMix64 + Box-Muller) ---- */
 typedef struct gt_gen_params {

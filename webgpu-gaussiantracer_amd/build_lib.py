@@ -31,7 +31,7 @@ CXX_FLAGS = [
 ]
 
 HIP_SOURCES = ["gt_forward.hip"]
-CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp"]
+CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp"]
 
 
 def _run(cmd, verbose):

@@ -43,7 +43,7 @@ __constant__ float c_normals[3 * GT_NUM_FACES];
 #ifndef GT_MIN_WAVES
 #define GT_MIN_WAVES 4 /* waves per SIMD the register budget is sized for */
 #endif
-#define GT_LDS_STACK 32 /* stack entries per lane held in LDS; deeper ones spill to scratch */
+#define GT_LDS_STACK 32 /* default LDS stack entries per lane; deeper ones spill to scratch */
 
 /* Traversal stack: the top GT_LDS_STACK entries live in LDS as a ring
  * (entry i at slot i % GT_LDS_STACK, lane-interleaved so that a wave's
@@ -51,18 +51,19 @@ __constant__ float c_normals[3 * GT_NUM_FACES];
 struct Stack {
   uint32_t* lds; /* this lane's column of the block's LDS stack */
   int sp;
+  int depth;     /* LDS entries per lane, a power of two */
   uint32_t spill[GT_STACK_MAX];
   __device__ __forceinline__ void push(uint32_t x) {
-    const int slot = (sp & (GT_LDS_STACK - 1)) * GT_BLOCK;
-    if (sp >= GT_LDS_STACK) spill[sp - GT_LDS_STACK] = lds[slot];
+    const int slot = (sp & (depth - 1)) * GT_BLOCK;
+    if (sp >= depth) spill[sp - depth] = lds[slot];
     lds[slot] = x;
     ++sp;
   }
   __device__ __forceinline__ uint32_t pop() {
     --sp;
-    const int slot = (sp & (GT_LDS_STACK - 1)) * GT_BLOCK;
+    const int slot = (sp & (depth - 1)) * GT_BLOCK;
     const uint32_t x = lds[slot];
-    if (sp >= GT_LDS_STACK) lds[slot] = spill[sp - GT_LDS_STACK];
+    if (sp >= depth) lds[slot] = spill[sp - depth];
     return x;
   }
 };
@@ -286,6 +287,15 @@ __device__ __forceinline__ gt_f3 radiance(const float* __restrict__ shp, uint32_
   return gt_f3_make(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f));
 }
 
+__device__ __forceinline__ uint32_t find_camera(const gt_dev_frame* cams, uint32_t n, uint32_t tile) {
+  uint32_t lo = 0, hi = n; /* last camera with tile_base <= tile */
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cams[mid].tile_base <= tile) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
   const uint32_t x = b & 7u, slot = b >> 3;
   const uint32_t q = nb >> 3, rem = nb & 7u;
@@ -298,23 +308,33 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-template <int MAXK, int MODE, bool DUMP, bool STATS>
-__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F) {
-  __shared__ uint32_t s_stack[GT_LDS_STACK * GT_BLOCK];
+/* One wave = one tile of tile_w x (64 / tile_w) pixels; block ids are
+ * remapped so that consecutive tiles share an XCD.  BATCH: global tiles of
+ * all cameras are numbered consecutively (camera c owns tiles from
+ * cams[c].tile_base). */
+template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH>
+__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
+                                                                    const gt_dev_frame* __restrict__ cams,
+                                                                    uint32_t n_cams, uint32_t total_tiles,
+                                                                    uint32_t tile_w, uint32_t lds_depth) {
+  extern __shared__ uint32_t s_stack[];
   __shared__ float s_faces[12 * GT_NUM_FACES];
   for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
   __syncthreads();
   Stack stk;
   stk.lds = s_stack + threadIdx.x;
   stk.sp = 0;
-  const uint32_t tiles_x = (F.width + 7u) >> 3;
-  const uint32_t tiles_y = (F.height + 7u) >> 3;
+  stk.depth = (int)lds_depth;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tile = blk * 4u + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
-  const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
-  const bool active = tile < tiles_x * tiles_y && px < F.width && py < F.height;
+  const uint32_t cam = (BATCH && tile < total_tiles) ? find_camera(cams, n_cams, tile) : 0u;
+  const gt_dev_frame& F = BATCH ? cams[cam] : F0;
+  const uint32_t tile_h = 64u / tile_w;
+  const uint32_t lt = tile - (BATCH ? F.tile_base : 0u);
+  const uint32_t px = (lt % F.tiles_x) * tile_w + (lane % tile_w);
+  const uint32_t py = (lt / F.tiles_x) * tile_h + (lane / tile_w);
+  const bool active = tile < total_tiles && px < F.width && py < F.height;
   Counters st;
   if (active) {
     const uint32_t ray = py * F.width + px;
@@ -493,23 +513,15 @@ __device__ __forceinline__ void visit_node(const gt_dev_scene& S, uint32_t ref, 
     if (rr[c] != GT_NODE_EMPTY) stk.push(rr[c]);
 }
 
-__device__ __forceinline__ uint32_t find_camera(const gt_dev_frame* cams, uint32_t n, uint32_t tile) {
-  uint32_t lo = 0, hi = n; /* last camera with tile_base <= tile */
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (cams[mid].tile_base <= tile) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 template <int MAXK, bool BATCH, bool DUMP, bool STATS>
 __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_scene S, const gt_dev_frame cam0,
                                                                    const gt_dev_frame* __restrict__ cams,
                                                                    uint32_t n_cams, uint32_t total_rays,
                                                                    unsigned int* work,
                                                                    unsigned long long* stats,
-                                                                   uint32_t REFILL_MIN, uint32_t COMP_MIN) {
-  __shared__ uint32_t s_stack[GT_LDS_STACK * GT_BLOCK];
+                                                                   uint32_t REFILL_MIN, uint32_t COMP_MIN,
+                                                                   uint32_t tile_w, uint32_t lds_depth) {
+  extern __shared__ uint32_t s_stack[];
   __shared__ float s_faces[12 * GT_NUM_FACES];
   for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
   __syncthreads();
@@ -517,6 +529,8 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
   Stack stk;
   stk.lds = s_stack + threadIdx.x;
   stk.sp = 0;
+  stk.depth = (int)lds_depth;
+  const uint32_t tile_h = 64u / tile_w;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t k = cam0.k;
@@ -583,7 +597,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
           const uint32_t tile = g >> 6, sub = g & 63u;
           if (BATCH) cam = find_camera(cams, n_cams, tile);
           const uint32_t lt = tile - CAM(tile_base), tx = CAM(tiles_x);
-          const uint32_t px = (lt % tx) * 8u + (sub & 7u), py = (lt / tx) * 8u + (sub >> 3);
+          const uint32_t px = (lt % tx) * tile_w + (sub % tile_w), py = (lt / tx) * tile_h + (sub / tile_w);
           const uint32_t W = CAM(width), H = CAM(height);
           if (px < W && py < H) {
             pix = py * W + px;
@@ -702,28 +716,34 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
 
 __global__ void k_reset_counter(unsigned int* work) { *work = 0u; }
 
-template <int MAXK, int MODE>
-hipError_t launch4(const gt_dev_scene& S, const gt_dev_frame& F, hipStream_t st, bool dump, bool stats) {
-  const uint32_t tiles = ((F.width + 7u) / 8u) * ((F.height + 7u) / 8u);
-  const dim3 grid((tiles + 3u) / 4u), block(GT_BLOCK);
-  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true>), grid, block, 0, st, S, F);
-  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false>), grid, block, 0, st, S, F);
-  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true>), grid, block, 0, st, S, F);
-  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false>), grid, block, 0, st, S, F);
+template <int MAXK, int MODE, bool BATCH>
+hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
+                        uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
+                        bool stats) {
+  const dim3 grid((total_tiles + 3u) / 4u), block(GT_BLOCK);
+  const size_t lds = (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
+#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth
+  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH>), grid, block, lds, st, GT_ARGS);
+  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH>), grid, block, lds, st, GT_ARGS);
+  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH>), grid, block, lds, st, GT_ARGS);
+  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false, BATCH>), grid, block, lds, st, GT_ARGS);
+#undef GT_ARGS
   return hipGetLastError();
 }
 
 template <int MAXK, bool BATCH>
 hipError_t launch_persistent(const gt_dev_scene& S, const gt_dev_frame& c0, const gt_dev_frame* cams, uint32_t n,
                              uint32_t total_rays, unsigned int* work, unsigned long long* stats, hipStream_t st,
-                             bool dump, int grid, uint32_t refill, uint32_t comp) {
+                             bool dump, int grid, uint32_t refill, uint32_t comp, uint32_t tile_w,
+                             uint32_t lds_depth) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, work);
   const dim3 g(grid), b(GT_BLOCK);
-#define GT_ARGS S, c0, cams, n, total_rays, work, stats, refill, comp
-  if (dump && stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, true>), g, b, 0, st, GT_ARGS);
-  else if (dump) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, false>), g, b, 0, st, GT_ARGS);
-  else if (stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, false, true>), g, b, 0, st, GT_ARGS);
-  else hipLaunchKernelGGL((k_render<MAXK, BATCH, false, false>), g, b, 0, st, GT_ARGS);
+  const size_t lds = (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
+#define GT_ARGS S, c0, cams, n, total_rays, work, stats, refill, comp, tile_w, lds_depth
+  if (dump && stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, true>), g, b, lds, st, GT_ARGS);
+  else if (dump) hipLaunchKernelGGL((k_render<MAXK, BATCH, true, false>), g, b, lds, st, GT_ARGS);
+  else if (stats) hipLaunchKernelGGL((k_render<MAXK, BATCH, false, true>), g, b, lds, st, GT_ARGS);
+  else hipLaunchKernelGGL((k_render<MAXK, BATCH, false, false>), g, b, lds, st, GT_ARGS);
 #undef GT_ARGS
   return hipGetLastError();
 }
@@ -768,7 +788,13 @@ hipError_t ensure_faces() {
 
 }  // namespace
 
-extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, void* stream) {
+static void tuning_defaults(const gt_tuning* t, uint32_t* tile_w, uint32_t* lds_depth) {
+  *tile_w = (t && t->tile_w) ? t->tile_w : 8u;
+  *lds_depth = (t && t->stack_lds) ? t->stack_lds : (uint32_t)GT_LDS_STACK;
+}
+
+extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, const gt_dev_frame* cams_dev,
+                                 uint32_t n_cams, uint32_t total_tiles, const gt_tuning* tune, void* stream) {
   hipError_t e = ensure_faces();
   if (e != hipSuccess) {
     gt_set_error("face table upload failed: %s", hipGetErrorString(e));
@@ -780,20 +806,27 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, v
   }
   const bool dump = F->hit_ids != nullptr && F->max_rounds > 0;
   const bool stats = F->stats != nullptr;
+  const bool batch = n_cams > 1;
+  uint32_t tile_w, depth;
+  tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
+#define GT_TILES(MK, MODE)                                                                                       \
+  (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats)  \
+         : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats))
   if (F->debug_mode == 1) {
-    e = launch4<1, MODE_CLOSEST>(*S, *F, st, dump, stats);
+    e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats);
   } else if (F->debug_mode == 2) {
-    e = launch4<1, MODE_COUNT>(*S, *F, st, dump, stats);
+    e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats);
   } else if (F->k <= 4) {
-    e = launch4<4, MODE_RENDER>(*S, *F, st, dump, stats);
+    e = GT_TILES(4, MODE_RENDER);
   } else if (F->k <= 8) {
-    e = launch4<8, MODE_RENDER>(*S, *F, st, dump, stats);
+    e = GT_TILES(8, MODE_RENDER);
   } else if (F->k <= 16) {
-    e = launch4<16, MODE_RENDER>(*S, *F, st, dump, stats);
+    e = GT_TILES(16, MODE_RENDER);
   } else {
-    e = launch4<32, MODE_RENDER>(*S, *F, st, dump, stats);
+    e = GT_TILES(32, MODE_RENDER);
   }
+#undef GT_TILES
   if (e != hipSuccess) {
     gt_set_error("forward launch failed: %s", hipGetErrorString(e));
     return GT_E_HIP;
@@ -824,11 +857,15 @@ extern "C" int gt_launch_persistent(const gt_dev_scene* S, const gt_dev_frame* c
   hipStream_t st = (hipStream_t)stream;
   const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_MIN_WAVES;
   const int grid = device_cus() * (int)bpc;
-  const uint32_t refill = tune && tune->refill_min ? tune->refill_min : 16u;
-  const uint32_t comp = tune && tune->comp_min ? tune->comp_min : 16u;
-#define GT_DISPATCH(MK)                                                                                              \
-  (batch ? launch_persistent<MK, true>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill, comp) \
-         : launch_persistent<MK, false>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill, comp))
+  const uint32_t refill = tune && tune->refill_min ? tune->refill_min : 48u;
+  const uint32_t comp = tune && tune->comp_min ? tune->comp_min : 48u;
+  uint32_t tile_w, depth;
+  tuning_defaults(tune, &tile_w, &depth);
+#define GT_DISPATCH(MK)                                                                                          \
+  (batch ? launch_persistent<MK, true>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill,   \
+                                       comp, tile_w, depth)                                                       \
+         : launch_persistent<MK, false>(*S, *cam0, cams_dev, n_cams, rays, work, stats, st, dump, grid, refill,  \
+                                        comp, tile_w, depth))
   if (k <= 4) e = GT_DISPATCH(4);
   else if (k <= 8) e = GT_DISPATCH(8);
   else if (k <= 16) e = GT_DISPATCH(16);

@@ -71,7 +71,11 @@ typedef struct gt_dev_frame {
 extern "C" {
 #endif
 /* implemented in gt_forward.hip */
-int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, void* stream);
+/* Tile renderer over n_cams cameras (frame F used directly when n_cams == 1,
+ * else the device array cams_dev with tile_base/tiles_x set for tune's
+ * tile shape); total_tiles = all cameras' tiles. */
+int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
+                      uint32_t total_tiles, const gt_tuning* tune, void* stream);
 /* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
  * array of n_cams frames (tile_base set, ascending); cam0: host copy of
  * frame 0, used directly when n_cams == 1.  work: one device counter the

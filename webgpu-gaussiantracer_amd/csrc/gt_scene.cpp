@@ -650,18 +650,31 @@ void gt_fill_dev_frame(const gt_uniforms* u, uint32_t width, uint32_t height, gt
 
 /* The per-tile kernel by default (faster on a single camera: the rays of a
  * wave stay coherent), the persistent kernel with tuning path 2. */
+uint32_t gt_tune_tile_w(const gt_tuning* t) { return t->tile_w ? t->tile_w : 8u; }
+
+/* tiles_x / tile_base of a frame for the tuning's tile shape; returns its tile count */
+uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base) {
+  const uint32_t tw = gt_tune_tile_w(t), th = 64u / tw;
+  f->tiles_x = (f->width + tw - 1u) / tw;
+  f->tile_base = tile_base;
+  return f->tiles_x * ((f->height + th - 1u) / th);
+}
+
 static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
-  if (f->debug_mode != 0 || s->tune.path != 2) return gt_launch_forward(&s->dev, f, stream);
-  const uint32_t tiles = f->tiles_x * ((f->height + 7u) / 8u);
+  const uint32_t tiles = gt_frame_tiles(&s->tune, f, 0);
+  if (f->debug_mode != 0 || s->tune.path != 2) return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream);
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
                               &s->tune, stream);
 }
 
 extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
   if (!s || !t) { gt_set_error("gt_scene_set_tuning: NULL argument"); return GT_E_INVALID; }
-  if (t->path > 2 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8) {
-    gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u)", t->path,
-                 t->refill_min, t->comp_min, t->blocks_per_cu);
+  const bool tile_ok = t->tile_w == 0 || t->tile_w == 4 || t->tile_w == 8 || t->tile_w == 16 || t->tile_w == 32;
+  const bool lds_ok = t->stack_lds == 0 || t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32 ||
+                      t->stack_lds == 64;
+  if (t->path > 2 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
+    gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u, tile_w %u, "
+                 "stack_lds %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w, t->stack_lds);
     return GT_E_INVALID;
   }
   s->tune = *t;
@@ -675,6 +688,8 @@ extern "C" int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t) {
 }
 
 const gt_dev_scene* gt_scene_dev(const gt_scene* s) { return &s->dev; }
+const gt_tuning* gt_scene_tuning(const gt_scene* s) { return &s->tune; }
+unsigned int* gt_scene_work(gt_scene* s) { return (unsigned int*)s->ctl.p; }
 
 extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
                          int buf_flags, void* stream) {

@@ -19,6 +19,14 @@ LIB_PATH = os.environ.get("GTRACE_LIB") or os.path.join(os.path.dirname(_HERE), 
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libgt.so not built at {LIB_PATH}; run __graft_entry__.build()")
+# One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 and
+# loads it under a different file name, so libgt (which needs the soname
+# libamdhip64.so.7) must be loaded after torch to bind to torch's copy;
+# torch streams and device pointers passed to libgt then belong to it.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
 _lib = C.CDLL(LIB_PATH)
 
 GT_OK = 0
@@ -65,7 +73,12 @@ class SceneInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_uint32), ("refill_min", C.c_uint32), ("comp_min", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("tile_w", C.c_uint32), ("stack_lds", C.c_uint32)]
+
+
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("best_ms", C.c_double), ("first_ms", C.c_double), ("n_tried", C.c_uint32),
+                ("_pad", C.c_uint32)]
 
 
 class GenParams(C.Structure):
@@ -95,6 +108,14 @@ _lib.gt_host_prepare.argtypes = [_P, C.c_uint32, C.c_double, _P, _P, _P, _P, C.P
 _lib.gt_debug_face_test.argtypes = [_P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float, _P, _P]
 _lib.gt_scene_set_tuning.argtypes = [_P, C.POINTER(Tuning)]
 _lib.gt_scene_get_tuning.argtypes = [_P, C.POINTER(Tuning)]
+_lib.gt_batch_create.argtypes = [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(_P)]
+_lib.gt_batch_render.argtypes = [_P, C.c_int, _P]
+_lib.gt_batch_rays.argtypes = [_P, C.POINTER(C.c_uint64)]
+_lib.gt_batch_destroy.argtypes = [_P]
+_lib.gt_batch_destroy.restype = None
+_lib.gt_autotune_frame.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, _P, C.c_uint32, _P,
+                                   C.POINTER(AutotuneResult)]
+_lib.gt_autotune_batch.argtypes = [_P, _P, C.c_int, C.c_uint32, _P, C.POINTER(AutotuneResult)]
 _lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
 _lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
 _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
@@ -105,7 +126,8 @@ _lib.gt_free.restype = None
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
+    "gt_render", "gt_autotune_frame", "gt_autotune_batch", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
+    "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
 
 
@@ -259,9 +281,10 @@ class Scene:
     def set_alpha_min(self, alpha_min):
         _check(_lib.gt_scene_set_alpha_min(self._h, float(alpha_min)))
 
-    def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0):
+    def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0):
         """Launch tuning (results do not depend on it); 0 = library default."""
-        _check(_lib.gt_scene_set_tuning(self._h, C.byref(Tuning(path, refill_min, comp_min, blocks_per_cu))))
+        _check(_lib.gt_scene_set_tuning(self._h, C.byref(Tuning(path, refill_min, comp_min, blocks_per_cu, tile_w,
+                                                                stack_lds))))
 
     def get_tuning(self):
         t = Tuning()
@@ -317,3 +340,65 @@ class Scene:
         f.rgba = rgba_ptr
         f.stats = stats_ptr
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 1, stream_ptr))
+
+
+class Batch:
+    """Many cameras rendered by one launch (optionally a replayed hipGraph).
+    frames: list of dicts with device pointers rgba (required) and optional
+    hit_ids, accept, n_rounds, T, max_rounds; stats_ptr accumulates counters."""
+
+    def __init__(self, scene, uniforms, sizes, frames, stats_ptr=None):
+        n = len(uniforms)
+        U = (Uniforms * n)(*uniforms)
+        Wa = (C.c_uint32 * n)(*[w for w, _ in sizes])
+        Ha = (C.c_uint32 * n)(*[h for _, h in sizes])
+        F = (Frame * n)()
+        for i, fr in enumerate(frames):
+            F[i].rgba = fr["rgba"]
+            F[i].hit_ids = fr.get("hit_ids")
+            F[i].accept = fr.get("accept")
+            F[i].n_rounds = fr.get("n_rounds")
+            F[i].T = fr.get("T")
+            F[i].max_rounds = fr.get("max_rounds", 0)
+        F[0].stats = stats_ptr
+        self._scene = scene  # keeps the scene alive
+        self._h = C.c_void_p()
+        _check(_lib.gt_batch_create(scene.handle, n, U, Wa, Ha, F, C.byref(self._h)))
+
+    def render(self, stream_ptr=None, use_graph=False):
+        _check(_lib.gt_batch_render(self._h, 1 if use_graph else 0, stream_ptr))
+
+    def rays(self):
+        r = C.c_uint64()
+        _check(_lib.gt_batch_rays(self._h, C.byref(r)))
+        return r.value
+
+    def close(self):
+        if self._h:
+            _lib.gt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _autotune_dict(r):
+    return dict(best={f: getattr(r.best, f) for f, _ in Tuning._fields_}, best_ms=r.best_ms, first_ms=r.first_ms,
+                n_tried=r.n_tried)
+
+
+def autotune_frame(scene, u, width, height, rgba_ptr, iters=3, stream_ptr=None):
+    """Picks the fastest launch tuning for this frame and stores it in the scene."""
+    r = AutotuneResult()
+    _check(_lib.gt_autotune_frame(scene.handle, C.byref(u), width, height, rgba_ptr, iters, stream_ptr, C.byref(r)))
+    return _autotune_dict(r)
+
+
+def autotune_batch(scene, batch, use_graph=True, iters=3, stream_ptr=None):
+    """Picks the fastest launch tuning for this batch and stores it in the scene."""
+    r = AutotuneResult()
+    _check(_lib.gt_autotune_batch(scene.handle, batch._h, 1 if use_graph else 0, iters, stream_ptr, C.byref(r)))
+    return _autotune_dict(r)
