@@ -172,7 +172,12 @@ typedef struct gt_frame {
   float* T;
   uint64_t* stats;
   uint32_t max_rounds;
-  uint32_t _pad;
+  uint32_t comp_cap; /* checkpoint capacity per ray (with comp_ids) */
+  /* Optional checkpoint for gt_backward: ids of the composited hits of ray
+   * y*W+x in order at comp_ids[(y*W+x)*comp_cap + i], their number at
+   * n_comp[y*W+x] (a count above comp_cap means the list overflowed). */
+  uint32_t* comp_ids;
+  uint32_t* n_comp;
 } gt_frame;
 
 /* traceRays(W, H) of the reference with the uniform block u.  Unlike the
@@ -180,6 +185,25 @@ typedef struct gt_frame {
  * limits: 1..16384 each, 1 <= k <= 32, t_min >= 0, sh_degree_max <= 3. */
 int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
               int buf_flags, void* hip_stream);
+
+/* ---- backward (no counterpart in the reference, which has no
+ * differentiation; README.md:68-69 plans training) ----
+ * Accumulates dLoss/d(raw parameters) of one frame into grads (n x 59
+ * floats, the PLY-raw layout of gt_scene_create_raw: mean, f_dc, f_rest,
+ * opacity logit, log-scale, raw quaternion) given dl_drgba (W*H float4,
+ * alpha channel ignored: A = 1 is constant).  Hit lists are held fixed.
+ * Checkpoint mode: comp_ids / n_comp / comp_cap from gt_render's outputs.
+ * Recompute mode (comp_ids NULL): the forward traversal is re-run in bands
+ * of rows whose scratch checkpoint stays under budget_bytes (0: 256 MiB),
+ * comp_cap (0: 256) bounding the composited hits per ray.  Rays whose list
+ * exceeds the capacity are skipped and counted in *overflow (device counter,
+ * optional).  All pointers are device pointers; checkpoint mode is
+ * asynchronous on the stream, recompute mode returns after the stream
+ * drains.  FP32 atomics: the summation order across rays is unspecified. */
+#define GT_GRAD_FLOATS 59
+int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const float* dl_drgba,
+                const uint32_t* comp_ids, const uint32_t* n_comp, uint32_t comp_cap, float* grads,
+                unsigned int* overflow, uint64_t budget_bytes, void* hip_stream);
 
 /* ---- batched cameras (no counterpart in the reference, which renders one
  * camera per traceRays dispatch, pass_encoder.ts:35-65) ---- */

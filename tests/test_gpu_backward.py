@@ -1,0 +1,218 @@
+"""GPU tests of the hand-written backward kernel (gt_backward).
+
+Reference: tests/fp64_render.py, an FP64 torch re-render of the same
+composited hit lists in the reference's covariance form, differentiated by
+torch autograd and by central finite differences (step 1e-4).  The hit
+lists are held fixed (the forward's list changes have no derivative).
+Bar: per parameter group relative L2 error <= 1e-3 and per element
+|g - ref| <= 1e-3 (|ref| + 1e-3 max|ref_group|).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import uniforms
+import fp64_render as fr
+import gtrace as gt
+import gtrace.autograd as gta
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = {"mean": slice(0, 3), "f_dc": slice(3, 6), "f_rest": slice(6, 51), "logit": slice(51, 52),
+          "log_scale": slice(52, 55), "quat": slice(55, 59)}
+REL_TOL = 1e-3
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _case(n=3000, W=48, H=40, seed=3, k=8, sh=3, alpha_min=0.05, pos=(0.3, -0.2, 5.0), **gen):
+    raw = gt.generate_raw(n=n, seed=seed, config="c1", rest_std=0.2, **gen)
+    s = gt.Scene(raw, alpha_min=alpha_min)
+    u = uniforms(list(pos), [0, 0, 0], W, H, hfov_deg=50, alpha_min=alpha_min, k=k, T_min=1e-3, sh_degree_max=sh)
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((H, W, 4)).astype(np.float32)
+    G[..., 3] = 0.0
+    return raw, s, u, G
+
+
+def _gpu_grads(s, n, u, W, H, G, mode="checkpoint", cap=256, budget=0):
+    dev = torch.device("cuda", 0)
+    g = torch.from_numpy(G).to(dev)
+    grads = torch.zeros((n, 59), dtype=torch.float32, device=dev)
+    ovf = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    if mode == "checkpoint":
+        rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+        cids = torch.empty((H * W, cap), dtype=torch.int32, device=dev)
+        nc = torch.empty(H * W, dtype=torch.int32, device=dev)
+        s.render_device(u, W, H, rgba.data_ptr(), st, comp_ids_ptr=cids.data_ptr(), n_comp_ptr=nc.data_ptr(),
+                        comp_cap=cap)
+        gt.backward(s, u, W, H, g.data_ptr(), grads.data_ptr(), cids.data_ptr(), nc.data_ptr(), cap, ovf.data_ptr(),
+                    0, st)
+    else:
+        gt.backward(s, u, W, H, g.data_ptr(), grads.data_ptr(), None, None, cap, ovf.data_ptr(), budget, st)
+    torch.cuda.synchronize()
+    return grads.cpu().numpy().astype(np.float64), int(ovf.cpu().item())
+
+
+def _reference(raw, s, u, W, H, G, sh):
+    """(fp64 pixels, autograd grads, checkpoint, closure f(raw64)->loss)."""
+    ck = s.render(u, W, H, checkpoint_cap=256)
+    R = W * H
+    rays, gids, pos = fr.flatten_checkpoint(ck["comp_ids"], ck["n_comp"])
+    p = orc.params_from_uniforms(u, W, H)
+    dirs = torch.from_numpy(np.stack([orc.ray_dir(p, r % W, r // W) for r in range(R)]).astype(np.float64))
+    o = torch.tensor([u.to_world[12], u.to_world[13], u.to_world[14]], dtype=torch.float64)
+    Gt = torch.from_numpy(G.reshape(R, 4)[:, :3].astype(np.float64))
+
+    def loss_of(raw64):
+        out = fr.render(raw64, o, dirs, rays, gids, pos, R, sh_degree=sh)
+        return (out * Gt).sum(), out
+
+    raw_t = torch.from_numpy(raw.astype(np.float64)).requires_grad_(True)
+    loss, out = loss_of(raw_t)
+    loss.backward()
+    return out.detach().numpy(), raw_t.grad.numpy(), ck, loss_of
+
+
+def _check_grads(g, ref, label):
+    for name, sl in GROUPS.items():
+        a, b = g[:, sl], ref[:, sl]
+        nb = np.linalg.norm(b)
+        if nb == 0:
+            assert np.abs(a).max() == 0, f"{label} {name}: nonzero gradient where the reference is zero"
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        scale = np.abs(b).max()
+        el = np.abs(a - b) / (np.abs(b) + 1e-3 * scale)
+        _print(f"  {label} {name}: rel L2 {rel:.2e}, worst element ratio {el.max():.2e}")
+        assert rel <= REL_TOL, f"{label} {name}: relative L2 error {rel:.3e}"
+        assert el.max() <= REL_TOL, f"{label} {name}: element error {el.max():.3e}"
+
+
+def test_checkpoint_matches_oracle_hit_lists():
+    """The composited ids of the checkpoint are the oracle's accepted hits, in order."""
+    _print("case backward checkpoint vs oracle 48x40 k8")
+    raw, s, u, _ = _case()
+    W, H = 48, 40
+    ck = s.render(u, W, H, checkpoint_cap=256)
+    o = orc.OracleScene(raw, 0.05)
+    r = orc.render(o, orc.params_from_uniforms(u, W, H), max_rounds=64)
+    assert r["stats"]["overflow"] == 0
+    bits = (r["accept"][:, :, None] >> np.arange(u.k, dtype=np.uint32)) & 1
+    for ray in range(W * H):
+        want = r["hit_ids"][ray][bits[ray].astype(bool)]
+        n = int(ck["n_comp"][ray])
+        assert n == len(want), f"ray {ray}: {n} composited, oracle {len(want)}"
+        np.testing.assert_array_equal(ck["comp_ids"][ray, :n], want, err_msg=f"ray {ray}")
+    assert int(ck["n_comp"].sum()) == r["stats"]["h_comp"] > 500
+
+
+@pytest.mark.parametrize("sh", [3, 1])
+def test_backward_vs_fp64_autograd(sh):
+    _print(f"case backward vs fp64 autograd sh{sh}")
+    raw, s, u, G = _case(sh=sh)
+    W, H = 48, 40
+    rgb64, ref, ck, _ = _reference(raw, s, u, W, H, G, sh)
+    rgba = s.render(u, W, H)["rgba"].reshape(-1, 4)
+    err = np.abs(rgba[:, :3] - rgb64).max()
+    _print(f"  fp64 re-render vs kernel pixels: {err:.2e}")
+    assert err <= 1e-4
+    g, ovf = _gpu_grads(s, raw.shape[0], u, W, H, G)
+    assert ovf == 0
+    _check_grads(g, ref, f"sh{sh}")
+    if sh == 1:
+        used = [6 + c * 15 + j for c in range(3) for j in range(3)]
+        unused = [c for c in range(6, 51) if c not in used]
+        assert np.all(g[:, unused] == 0) and np.any(g[:, used] != 0)
+
+
+def test_backward_vs_central_differences():
+    """512 parameters of Gaussians on the hit lists, central FD of the FP64 re-render, step 1e-4."""
+    _print("case backward vs central FD 512 params")
+    raw, s, u, G = _case()
+    W, H = 48, 40
+    _, ref, ck, loss_of = _reference(raw, s, u, W, H, G, 3)
+    g, _ = _gpu_grads(s, raw.shape[0], u, W, H, G)
+    hit = np.unique(np.concatenate([ck["comp_ids"][r, :ck["n_comp"][r]] for r in range(W * H)]))
+    rng = np.random.default_rng(7)
+    picks = []
+    per = {"mean": 96, "f_dc": 64, "f_rest": 96, "logit": 64, "log_scale": 96, "quat": 96}
+    for name, cnt in per.items():
+        sl = GROUPS[name]
+        cols = np.arange(sl.start, sl.stop)
+        picks += [(int(rng.choice(hit)), int(rng.choice(cols))) for _ in range(cnt)]
+    assert len(picks) == 512
+    raw64 = torch.from_numpy(raw.astype(np.float64))
+    h = 1e-4
+    fd = np.zeros(len(picks))
+    with torch.no_grad():
+        for i, (gid, col) in enumerate(picks):
+            rp = raw64.clone()
+            rp[gid, col] += h
+            rm = raw64.clone()
+            rm[gid, col] -= h
+            fd[i] = (loss_of(rp)[0].item() - loss_of(rm)[0].item()) / (2 * h)
+    gv = np.array([g[a, b] for a, b in picks])
+    av = np.array([ref[a, b] for a, b in picks])
+    scale = np.abs(fd).max()
+    tol = 1e-3 * (np.abs(fd) + 1e-2 * scale)
+    bad = np.nonzero(np.abs(gv - fd) > tol)[0]
+    _print(f"  FD max |kernel-fd| {np.abs(gv - fd).max():.2e}, |autograd-fd| {np.abs(av - fd).max():.2e}, "
+           f"max|fd| {scale:.2e}")
+    assert bad.size == 0, f"{bad.size} of 512 differ, first {[picks[i] for i in bad[:4]]} {gv[bad[:4]]} {fd[bad[:4]]}"
+    assert np.all(np.abs(av - fd) <= tol)
+
+
+def test_recompute_equals_checkpoint():
+    """Recompute mode over several bands (small scratch budget) == checkpoint mode up to atomic order."""
+    _print("case backward recompute vs checkpoint")
+    raw, s, u, G = _case(W=72, H=56)
+    W, H = 72, 56
+    n = raw.shape[0]
+    a, _ = _gpu_grads(s, n, u, W, H, G, "checkpoint")
+    per_row = W * 8 * 257 * 4
+    for budget in (2 * per_row, 0):
+        b, ovf = _gpu_grads(s, n, u, W, H, G, "recompute", budget=budget)
+        assert ovf == 0
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6 * np.abs(a).max())
+
+
+def test_checkpoint_overflow_is_reported():
+    _print("case backward overflow")
+    raw, s, u, G = _case()
+    W, H = 48, 40
+    ck = s.render(u, W, H, checkpoint_cap=256)
+    _, ovf = _gpu_grads(s, raw.shape[0], u, W, H, G, cap=2)
+    assert ovf == int((ck["n_comp"] > 2).sum()) > 0
+
+
+def test_autograd_function():
+    _print("case torch.autograd wrapper")
+    raw, s, u, G = _case()
+    W, H = 48, 40
+    dev = torch.device("cuda", 0)
+    want, _ = _gpu_grads(s, raw.shape[0], u, W, H, G)
+    rgba_host = s.render(u, W, H)["rgba"]
+    for mode in ("checkpoint", "recompute"):
+        x = torch.from_numpy(raw).to(dev).requires_grad_(True)
+        rgba = gta.render(x, u, W, H, 0.05, mode=mode)
+        np.testing.assert_array_equal(rgba.detach().cpu().numpy().reshape(rgba_host.shape), rgba_host)
+        (rgba * torch.from_numpy(G).to(dev)).sum().backward()
+        np.testing.assert_allclose(x.grad.cpu().numpy(), want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+
+
+def test_backward_rejects_bad_arguments():
+    raw, s, u, G = _case(n=200)
+    dev = torch.device("cuda", 0)
+    g = torch.zeros((40, 48, 4), device=dev)
+    grads = torch.zeros((200, 59), device=dev)
+    cids = torch.zeros((48 * 40, 4), dtype=torch.int32, device=dev)
+    with pytest.raises(gt.GTError):
+        gt.backward(s, u, 48, 40, g.data_ptr(), grads.data_ptr(), cids.data_ptr(), None, 4)
+    u2 = uniforms([0, 0, 5], [0, 0, 0], 48, 40, debug_mode=1)
+    with pytest.raises(gt.GTError):
+        gt.backward(s, u2, 48, 40, g.data_ptr(), grads.data_ptr())

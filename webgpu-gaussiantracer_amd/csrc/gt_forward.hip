@@ -316,7 +316,8 @@ template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH>
 __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
                                                                     const gt_dev_frame* __restrict__ cams,
                                                                     uint32_t n_cams, uint32_t total_tiles,
-                                                                    uint32_t tile_w, uint32_t lds_depth) {
+                                                                    uint32_t tile_w, uint32_t lds_depth,
+                                                                    uint32_t tile_offset) {
   extern __shared__ uint32_t s_stack[];
   __shared__ float s_faces[12 * GT_NUM_FACES];
   for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
   stk.sp = 0;
   stk.depth = (int)lds_depth;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t tile = blk * 4u + (threadIdx.x >> 6);
+  const uint32_t tile = tile_offset + blk * 4u + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t cam = (BATCH && tile < total_tiles) ? find_camera(cams, n_cams, tile) : 0u;
   const gt_dev_frame& F = BATCH ? cams[cam] : F0;
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
     }
     gt_f3 L = gt_f3_make(0.0f, 0.0f, 0.0f);
     float T = 1.0f;
-    uint32_t rounds = 0;
+    uint32_t rounds = 0, n_comp = 0;
     const size_t MR = F.max_rounds;
     KBuf<MAXK> kb;
     uint32_t count = 0;
@@ -395,6 +396,10 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
                 T *= 1.0f - alpha;
                 mask |= 1u << j;
                 st.h_comp++;
+                if (F.comp_ids) {
+                  if (n_comp < F.comp_cap) F.comp_ids[(size_t)(ray - F.comp_base) * F.comp_cap + n_comp] = id;
+                  ++n_comp;
+                }
               }
             }
           }
@@ -438,6 +443,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
     }
     reinterpret_cast<float4*>(F.rgba)[ray] = make_float4(L.x, L.y, L.z, 1.0f);
     if (F.n_rounds) F.n_rounds[ray] = rounds;
+    if (F.n_comp) F.n_comp[ray - F.comp_base] = n_comp;
     if (F.T_out) F.T_out[ray] = T;
     if (DUMP) {
       const uint32_t kk = MODE == MODE_RENDER ? F.k : F.k;
@@ -719,10 +725,10 @@ __global__ void k_reset_counter(unsigned int* work) { *work = 0u; }
 template <int MAXK, int MODE, bool BATCH>
 hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                         uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                        bool stats) {
-  const dim3 grid((total_tiles + 3u) / 4u), block(GT_BLOCK);
+                        bool stats, uint32_t tile_offset) {
+  const dim3 grid((total_tiles - tile_offset + 3u) / 4u), block(GT_BLOCK);
   const size_t lds = (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
-#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth
+#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset
   if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH>), grid, block, lds, st, GT_ARGS);
   else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH>), grid, block, lds, st, GT_ARGS);
   else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH>), grid, block, lds, st, GT_ARGS);
@@ -794,7 +800,9 @@ static void tuning_defaults(const gt_tuning* t, uint32_t* tile_w, uint32_t* lds_
 }
 
 extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, const gt_dev_frame* cams_dev,
-                                 uint32_t n_cams, uint32_t total_tiles, const gt_tuning* tune, void* stream) {
+                                 uint32_t n_cams, uint32_t total_tiles, const gt_tuning* tune, void* stream,
+                                 uint32_t tile_offset) {
+  if (tile_offset >= total_tiles) return GT_OK;
   hipError_t e = ensure_faces();
   if (e != hipSuccess) {
     gt_set_error("face table upload failed: %s", hipGetErrorString(e));
@@ -811,12 +819,16 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
 #define GT_TILES(MK, MODE)                                                                                       \
-  (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats)  \
-         : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats))
+  (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
+                                       tile_offset)                                                              \
+         : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
+                                        tile_offset))
   if (F->debug_mode == 1) {
-    e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats);
+    e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
+                                             tile_offset);
   } else if (F->debug_mode == 2) {
-    e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats);
+    e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
+                                           tile_offset);
   } else if (F->k <= 4) {
     e = GT_TILES(4, MODE_RENDER);
   } else if (F->k <= 8) {

@@ -59,6 +59,13 @@ typedef struct gt_dev_frame {
   uint32_t max_rounds;
   uint32_t tiles_x;   /* 8x8 pixel tiles per row */
   uint32_t tile_base; /* first global tile index of this camera in a batch */
+  /* optional checkpoint for the backward pass: ids of the composited hits of
+   * pixel p (in order) at comp_ids[(p - comp_base) * comp_cap + i], their
+   * number at n_comp[p - comp_base] (may exceed comp_cap: overflow) */
+  uint32_t* comp_ids;
+  uint32_t* n_comp;
+  uint32_t comp_cap;
+  uint32_t comp_base;
   float* rgba;
   uint32_t* hit_ids;
   uint32_t* accept;
@@ -75,7 +82,7 @@ extern "C" {
  * else the device array cams_dev with tile_base/tiles_x set for tune's
  * tile shape); total_tiles = all cameras' tiles. */
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
-                      uint32_t total_tiles, const gt_tuning* tune, void* stream);
+                      uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset);
 /* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
  * array of n_cams frames (tile_base set, ascending); cam0: host copy of
  * frame 0, used directly when n_cams == 1.  work: one device counter the

@@ -470,7 +470,13 @@ static int build_and_upload(gt_scene* s) {
     const float* o = &h.packed[(size_t)i * 80];
     float* g = &geom[(size_t)i * 12];
     g[0] = o[0]; g[1] = o[1]; g[2] = o[2]; g[3] = o[12];
-    g[4] = o[4]; g[5] = o[5]; g[6] = o[6]; g[7] = 0.0f;
+    /* g[7]: |raw quaternion|, for the adjoint of the normalisation (1 for packed input) */
+    float qn = 1.0f;
+    if (!h.raw.empty()) {
+      const float* r = &h.raw[(size_t)i * GT_RAW_FLOATS + 55];
+      qn = (float)std::sqrt((((double)r[0] * r[0] + (double)r[1] * r[1]) + (double)r[2] * r[2]) + (double)r[3] * r[3]);
+    }
+    g[4] = o[4]; g[5] = o[5]; g[6] = o[6]; g[7] = qn;
     g[8] = o[8]; g[9] = o[9]; g[10] = o[10]; g[11] = o[11];
     for (int j = 0; j < 16; ++j)
       for (int c = 0; c < 3; ++c) sh[(size_t)i * 48 + 3 * j + c] = o[16 + 4 * j + c];
@@ -662,7 +668,9 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
 
 static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   const uint32_t tiles = gt_frame_tiles(&s->tune, f, 0);
-  if (f->debug_mode != 0 || s->tune.path != 2) return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream);
+  /* the checkpoint for the backward pass is written by the tile kernel */
+  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids)
+    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream, 0);
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
                               &s->tune, stream);
 }
@@ -700,6 +708,14 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     gt_set_error("gt_render: max_rounds must be >= 1 when hit_ids/accept are requested");
     return GT_E_INVALID;
   }
+  if ((out->comp_ids != nullptr) != (out->n_comp != nullptr) || (out->comp_ids && out->comp_cap == 0)) {
+    gt_set_error("gt_render: the checkpoint needs comp_ids, n_comp and comp_cap >= 1");
+    return GT_E_INVALID;
+  }
+  if (out->comp_ids && u->debug_mode != 0) {
+    gt_set_error("gt_render: the checkpoint is written in debugMode 0 only");
+    return GT_E_INVALID;
+  }
   gt_dev_frame f;
   gt_fill_dev_frame(u, width, height, &f);
   const size_t R = (size_t)width * height;
@@ -712,13 +728,16 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     f.T_out = out->T;
     f.stats = (unsigned long long*)out->stats;
     f.max_rounds = (uint32_t)MR;
+    f.comp_ids = out->comp_ids;
+    f.n_comp = out->n_comp;
+    f.comp_cap = out->comp_cap;
     return launch_frame(s, &f, stream);
   }
   // host buffers: device temporaries, synchronous
   struct Tmp {
     void* p = nullptr;
     ~Tmp() { if (p) (void)hipFree(p); }
-  } t_rgba, t_hits, t_acc, t_rounds, t_T, t_stats;
+  } t_rgba, t_hits, t_acc, t_rounds, t_T, t_stats, t_cids, t_ncomp;
   auto alloc = [&](Tmp& t, size_t bytes) -> int {
     hipError_t e = hipMalloc(&t.p, std::max<size_t>(bytes, 4));
     if (e != hipSuccess) {
@@ -751,6 +770,13 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     HIP_TRY(hipMemset(t_stats.p, 0, GT_NSTATS * 8));
     f.stats = (unsigned long long*)t_stats.p;
   }
+  if (out->comp_ids) {
+    if ((rc = alloc(t_cids, R * out->comp_cap * 4)) != GT_OK) return rc;
+    if ((rc = alloc(t_ncomp, R * 4)) != GT_OK) return rc;
+    f.comp_ids = (uint32_t*)t_cids.p;
+    f.n_comp = (uint32_t*)t_ncomp.p;
+    f.comp_cap = out->comp_cap;
+  }
   f.max_rounds = (uint32_t)MR;
   if ((rc = launch_frame(s, &f, nullptr)) != GT_OK) return rc;
   HIP_TRY(hipDeviceSynchronize());
@@ -760,6 +786,10 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
   if (out->n_rounds) HIP_TRY(hipMemcpy(out->n_rounds, f.n_rounds, R * 4, hipMemcpyDeviceToHost));
   if (out->T) HIP_TRY(hipMemcpy(out->T, f.T_out, R * 4, hipMemcpyDeviceToHost));
   if (out->stats) HIP_TRY(hipMemcpy(out->stats, f.stats, GT_NSTATS * 8, hipMemcpyDeviceToHost));
+  if (out->comp_ids) {
+    HIP_TRY(hipMemcpy(out->comp_ids, f.comp_ids, R * out->comp_cap * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out->n_comp, f.n_comp, R * 4, hipMemcpyDeviceToHost));
+  }
   return GT_OK;
 }
 

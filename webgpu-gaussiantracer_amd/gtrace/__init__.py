@@ -59,7 +59,7 @@ class Frame(C.Structure):
     _fields_ = [
         ("rgba", C.c_void_p), ("hit_ids", C.c_void_p), ("accept", C.c_void_p),
         ("n_rounds", C.c_void_p), ("T", C.c_void_p), ("stats", C.c_void_p),
-        ("max_rounds", C.c_uint32), ("_pad", C.c_uint32),
+        ("max_rounds", C.c_uint32), ("comp_cap", C.c_uint32), ("comp_ids", C.c_void_p), ("n_comp", C.c_void_p),
     ]
 
 
@@ -116,6 +116,8 @@ _lib.gt_batch_destroy.restype = None
 _lib.gt_autotune_frame.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, _P, C.c_uint32, _P,
                                    C.POINTER(AutotuneResult)]
 _lib.gt_autotune_batch.argtypes = [_P, _P, C.c_int, C.c_uint32, _P, C.POINTER(AutotuneResult)]
+_lib.gt_backward.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P,
+                             C.c_uint64, _P]
 _lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
 _lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
 _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
@@ -126,7 +128,7 @@ _lib.gt_free.restype = None
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_autotune_frame", "gt_autotune_batch", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
+    "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
 
@@ -306,9 +308,10 @@ class Scene:
         _check(_lib.gt_scene_get_host_data(self._h, _ptr(packed), _ptr(ids), _ptr(o2w), _ptr(w2o)))
         return dict(packed=packed, proxy_ids=ids, o2w=o2w, w2o=w2o)
 
-    def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False):
+    def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False, checkpoint_cap=0):
         """traceRays(W, H) with host buffers: returns rgba (H, W, 4) and, on request,
-        hit lists (H*W, max_rounds, k), accept masks, round counts, T and stats."""
+        hit lists (H*W, max_rounds, k), accept masks, round counts, T, stats and
+        the backward checkpoint (composited ids per ray, capacity checkpoint_cap)."""
         R = width * height
         rgba = np.zeros((height, width, 4), np.float32)
         out = {"rgba": rgba}
@@ -329,16 +332,24 @@ class Scene:
             st = np.zeros(GT_NSTATS, np.uint64)
             f.stats = _ptr(st)
             out["stats"] = st
+        if checkpoint_cap:
+            cids = np.zeros((R, checkpoint_cap), np.uint32)
+            ncomp = np.zeros(R, np.uint32)
+            f.comp_ids, f.n_comp, f.comp_cap = _ptr(cids), _ptr(ncomp), checkpoint_cap
+            out.update(comp_ids=cids, n_comp=ncomp)
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 0, None))
         if want_stats:
             out["stats"] = dict(zip(STAT_NAMES, [int(x) for x in out["stats"]]))
         return out
 
-    def render_device(self, u, width, height, rgba_ptr, stream_ptr=None, stats_ptr=None):
-        """Asynchronous traceRays into caller-owned device memory (GT_BUF_DEVICE)."""
+    def render_device(self, u, width, height, rgba_ptr, stream_ptr=None, stats_ptr=None, comp_ids_ptr=None,
+                      n_comp_ptr=None, comp_cap=0):
+        """Asynchronous traceRays into caller-owned device memory (GT_BUF_DEVICE),
+        optionally writing the backward checkpoint."""
         f = Frame()
         f.rgba = rgba_ptr
         f.stats = stats_ptr
+        f.comp_ids, f.n_comp, f.comp_cap = comp_ids_ptr, n_comp_ptr, comp_cap
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 1, stream_ptr))
 
 
@@ -402,3 +413,10 @@ def autotune_batch(scene, batch, use_graph=True, iters=3, stream_ptr=None):
     r = AutotuneResult()
     _check(_lib.gt_autotune_batch(scene.handle, batch._h, 1 if use_graph else 0, iters, stream_ptr, C.byref(r)))
     return _autotune_dict(r)
+
+
+def backward(scene, u, width, height, dl_drgba_ptr, grads_ptr, comp_ids_ptr=None, n_comp_ptr=None, comp_cap=0,
+             overflow_ptr=None, budget_bytes=0, stream_ptr=None):
+    """Accumulates dLoss/draw (n x 59, device) for one frame; device pointers."""
+    _check(_lib.gt_backward(scene.handle, C.byref(u), width, height, dl_drgba_ptr, comp_ids_ptr, n_comp_ptr,
+                            comp_cap, grads_ptr, overflow_ptr, budget_bytes, stream_ptr))
