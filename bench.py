@@ -16,8 +16,13 @@ RGBA write, the 44-B mu/scale/rotation/opacity read per evaluated hit and
 the 192-B SH read per composited hit), divided by the kernel's average
 duration measured with HIP events on the launch stream.
 
+fwd_bwd: forward with the backward checkpoint + dL/dRGBA of a squared-error
+loss + the hand-written backward, on the same frame (and the recompute mode).
+batched: 64 cameras of uneven resolution in one launch replayed from a
+hipGraph, against the same cameras rendered one launch at a time.
+
 cpu_baseline: the oracle (the project's C restatement of the reference path,
-BVH variant, all host threads up to 16) on a seeded 1/64 sample of the rays.
+BVH variant, all host threads up to 16) on a seeded 1/3 sample of the rays.
 """
 import argparse
 import json
@@ -98,6 +103,133 @@ def bench_batched(gt, scene, torch, dev, sp, K, n_cams, steps=3):
     }
 
 
+class MemSampler:
+    """Peak device memory in use (hipMemGetInfo: total - free), sampled every ~1 ms on a thread."""
+
+    def __init__(self, torch):
+        import threading
+        self.torch, self.peak, self._stop = torch, 0, threading.Event()
+        self._t = threading.Thread(target=self._run, daemon=True)
+
+    def _run(self):
+        while not self._stop.is_set():
+            free, total = self.torch.cuda.mem_get_info()
+            self.peak = max(self.peak, total - free)
+            self._stop.wait(0.001)
+
+    def __enter__(self):
+        self._t.start()
+        return self
+
+    def __exit__(self, *a):
+        self._stop.set()
+        self._t.join()
+        free, total = self.torch.cuda.mem_get_info()
+        self.peak = max(self.peak, total - free)
+
+
+def _events_ms(torch, stream, fn, n):
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(n):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd, st_fwd):
+    """Forward + backward of loss = sum (render - target)^2, target rendered from
+    the same scene with means perturbed by N(0, 0.01) (seed 3).  One step =
+    forward with the per-ray checkpoint of composited ids, dL/dRGBA =
+    2 (render - target), zeroing of the n x 59 gradient and gt_backward.  The
+    recompute mode (no checkpoint; traversal re-run inside the backward over
+    bands of a 256 MiB scratch) is timed beside it."""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    R = W * H
+    n = raw.shape[0]
+    rng = np.random.Generator(np.random.PCG64(3))
+    raw_t = raw.copy()
+    raw_t[:, 0:3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    target = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    st = gt.Scene(raw_t, alpha_min=0.01)
+    st.render_device(u, W, H, target.data_ptr(), sp)
+    torch.cuda.synchronize()
+    st.close()
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    gimg = torch.empty_like(rgba)
+    grads = torch.zeros((n, gt.GT_RAW_FLOATS), dtype=torch.float32, device=dev)
+    ovf = torch.zeros(1, dtype=torch.int32, device=dev)
+    ncomp = torch.empty(R, dtype=torch.int32, device=dev)
+    scene.render_device(u, W, H, rgba.data_ptr(), sp, n_comp_ptr=ncomp.data_ptr(), comp_ids_ptr=None)
+    torch.cuda.synchronize()
+    most = int(ncomp.max().item())
+    cap = 1 << max(0, most - 1).bit_length()
+    cids = torch.empty((R, cap), dtype=torch.int32, device=dev)
+
+    def fwd_ck():
+        scene.render_device(u, W, H, rgba.data_ptr(), sp, comp_ids_ptr=cids.data_ptr(),
+                            n_comp_ptr=ncomp.data_ptr(), comp_cap=cap)
+
+    def loss_grad():
+        torch.sub(rgba, target, out=gimg)
+        gimg.mul_(2.0)
+        grads.zero_()
+
+    def bwd():
+        gt.backward(scene, u, W, H, gimg.data_ptr(), grads.data_ptr(), cids.data_ptr(), ncomp.data_ptr(), cap,
+                    ovf.data_ptr(), 0, sp)
+
+    def step():
+        fwd_ck()
+        loss_grad()
+        bwd()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    with MemSampler(torch) as mem:
+        ms_fb = _events_ms(torch, stream, step, steps)
+    ms_fwd_ck = _events_ms(torch, stream, fwd_ck, steps)
+    ms_bwd = _events_ms(torch, stream, bwd, steps)
+    step()  # one clean step for the reported loss and gradient norms
+    torch.cuda.synchronize()
+    assert int(ovf.item()) == 0
+    loss = float(((rgba - target) ** 2).sum().item())
+    gn = grads.norm(dim=0)
+
+    def step_rc():
+        scene.render_device(u, W, H, rgba.data_ptr(), sp)
+        loss_grad()
+        gt.backward(scene, u, W, H, gimg.data_ptr(), grads.data_ptr(), None, None, 0, ovf.data_ptr(), 0, sp)
+
+    step_rc()
+    torch.cuda.synchronize()
+    rc_steps = max(1, steps // 4)
+    with MemSampler(torch) as mem_rc:
+        ms_rc = _events_ms(torch, stream, step_rc, rc_steps)
+    h_comp, h_eval = st_fwd["h_comp"], st_fwd["h_eval"]
+    B_fb = 16 * R + 44 * h_eval + 192 * h_comp + 16 * R + 44 * h_eval + (236 + 2 * 236) * h_comp
+    return {
+        "workload": f"loss = sum (render - target)^2, target from means + N(0, 0.01); forward with checkpoint, "
+                    f"dL/dRGBA, backward; {n} Gaussians, {W}x{H}, k {u.k}",
+        "mrays": round(R / ms_fb / 1e3, 3), "ms_per_step": round(ms_fb, 3),
+        "ratio_vs_forward": round(ms_fb / ms_fwd, 3),
+        "ms_forward_with_checkpoint": round(ms_fwd_ck, 3), "ms_backward": round(ms_bwd, 3),
+        "checkpoint_cap": cap, "checkpoint_bytes": R * cap * 4 + R * 4, "max_composited_per_ray": most,
+        "algorithmic_bytes": int(B_fb), "roofline_frac": round(B_fb / (ms_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "peak_device_bytes": int(mem.peak), "loss": loss,
+        "grad_norms": {"mean": float(gn[0:3].norm()), "sh": float(gn[3:51].norm()), "logit": float(gn[51]),
+                       "log_scale": float(gn[52:55].norm()), "quat": float(gn[55:59].norm())},
+        "recompute": {"ms_per_step": round(ms_rc, 3), "mrays": round(R / ms_rc / 1e3, 3),
+                      "ratio_vs_forward": round(ms_rc / ms_fwd, 3), "peak_device_bytes": int(mem_rc.peak),
+                      "steps": rc_steps},
+        "steps": steps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +242,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-camera measurement")
     ap.add_argument("--batch-cams", type=int, default=64)
+    ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
     ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -212,6 +345,9 @@ def main():
         "image_mean": float(img[..., :3].mean()),
     }
 
+    if not args.no_backward:
+        out["fwd_bwd"] = bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, max(2, args.steps // 2),
+                                       min(args.warmup, 2), ms_step, st)
     if not args.no_batched:
         out["batched"] = bench_batched(gt, scene, torch, dev, sp, K, args.batch_cams)
 

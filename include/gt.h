@@ -175,7 +175,8 @@ typedef struct gt_frame {
   uint32_t comp_cap; /* checkpoint capacity per ray (with comp_ids) */
   /* Optional checkpoint for gt_backward: ids of the composited hits of ray
    * y*W+x in order at comp_ids[(y*W+x)*comp_cap + i], their number at
-   * n_comp[y*W+x] (a count above comp_cap means the list overflowed). */
+   * n_comp[y*W+x] (a count above comp_cap means the list overflowed).
+   * n_comp without comp_ids only counts, to size a checkpoint. */
   uint32_t* comp_ids;
   uint32_t* n_comp;
 } gt_frame;
@@ -193,11 +194,16 @@ int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height
  * opacity logit, log-scale, raw quaternion) given dl_drgba (W*H float4,
  * alpha channel ignored: A = 1 is constant).  Hit lists are held fixed.
  * Checkpoint mode: comp_ids / n_comp / comp_cap from gt_render's outputs.
- * Recompute mode (comp_ids NULL): the forward traversal is re-run in bands
- * of rows whose scratch checkpoint stays under budget_bytes (0: 256 MiB),
- * comp_cap (0: 256) bounding the composited hits per ray.  Rays whose list
- * exceeds the capacity are skipped and counted in *overflow (device counter,
- * optional).  All pointers are device pointers; checkpoint mode is
+ * Rays whose list exceeds comp_cap are skipped and counted in *overflow
+ * (device counter, optional).  Recompute mode (comp_ids NULL): the forward
+ * traversal is re-run in bands of rows whose scratch checkpoint stays near
+ * budget_bytes (0: 1 GiB), starting with comp_cap (0: 64) ids per ray and
+ * growing it (with smaller bands, down to 8 rows) when a ray composites
+ * more, so no ray overflows; the scratch (about budget_bytes + 20 B per
+ * pixel) and the capacity stay with the scene until gt_scene_destroy.
+ * Both modes also keep 80 B per Gaussian of accumulators with the scene,
+ * so backward calls on one scene must not overlap.  All pointers are
+ * device pointers; checkpoint mode is
  * asynchronous on the stream, recompute mode returns after the stream
  * drains.  FP32 atomics: the summation order across rays is unspecified. */
 #define GT_GRAD_FLOATS 59

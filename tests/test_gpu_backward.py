@@ -175,8 +175,9 @@ def test_recompute_equals_checkpoint():
     n = raw.shape[0]
     a, _ = _gpu_grads(s, n, u, W, H, G, "checkpoint")
     per_row = W * 8 * 257 * 4
-    for budget in (2 * per_row, 0):
-        b, ovf = _gpu_grads(s, n, u, W, H, G, "recompute", budget=budget)
+    for budget, cap in ((2 * per_row, 256), (0, 256), (W * 8 * 2 * 4, 2)):
+        # the last case starts with room for 2 ids per ray: the scratch must grow, never overflow
+        b, ovf = _gpu_grads(s, n, u, W, H, G, "recompute", cap=cap, budget=budget)
         assert ovf == 0
         np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6 * np.abs(a).max())
 
@@ -190,8 +191,9 @@ def test_checkpoint_overflow_is_reported():
     assert ovf == int((ck["n_comp"] > 2).sum()) > 0
 
 
-def test_autograd_function():
+def test_autograd_function(monkeypatch):
     _print("case torch.autograd wrapper")
+    monkeypatch.setattr(gta, "CHECKPOINT_CAP", 2)  # forces the re-render with a larger checkpoint
     raw, s, u, G = _case()
     W, H = 48, 40
     dev = torch.device("cuda", 0)
@@ -216,3 +218,13 @@ def test_backward_rejects_bad_arguments():
     u2 = uniforms([0, 0, 5], [0, 0, 0], 48, 40, debug_mode=1)
     with pytest.raises(gt.GTError):
         gt.backward(s, u2, 48, 40, g.data_ptr(), grads.data_ptr())
+
+
+def test_count_only_checkpoint():
+    """n_comp without comp_ids counts the composited hits (to size a checkpoint)."""
+    _print("case count-only checkpoint")
+    raw, s, u, _ = _case()
+    full = s.render(u, 48, 40, checkpoint_cap=256)
+    cnt = s.render(u, 48, 40, checkpoint_cap=-1)
+    np.testing.assert_array_equal(cnt["n_comp"], full["n_comp"])
+    np.testing.assert_array_equal(cnt["rgba"], full["rgba"])

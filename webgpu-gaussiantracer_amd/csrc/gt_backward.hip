@@ -32,6 +32,8 @@
 #include "gt_internal.h"
 
 /* gt_scene.cpp */
+void* gt_scene_bwd_scratch(gt_scene* s, int which, size_t bytes);
+uint32_t* gt_scene_rc_cap(gt_scene* s);
 const gt_dev_scene* gt_scene_dev(const gt_scene* s);
 const gt_tuning* gt_scene_tuning(const gt_scene* s);
 int gt_validate_uniforms(const gt_uniforms* u, uint32_t width, uint32_t height);
@@ -79,30 +81,47 @@ __device__ __forceinline__ void sh_basis(double x, double y, double z, double* b
   b[15] = SH_C3_6 * x * (xx - 3 * yy); db[15][0] = SH_C3_6 * (3 * xx - 3 * yy); db[15][1] = -6 * SH_C3_6 * x * y; db[15][2] = 0;
 }
 
-/* Everything the two passes need about one hit. */
-struct Hit {
+/* Per-Gaussian accumulators of one frame (GT_ACC floats each).  All rays
+ * of a frame share the camera origin o, so the colour c and its SH basis
+ * are per-Gaussian constants; a hit contributes
+ *   [0..2]   G_ch T alpha                 (dLoss/dc before the clamp)
+ *   [3]      dLoss/dalpha * response      (opacity logit)
+ *   [4..6]   dLoss/dmu through the response
+ *   [7..9]   dLoss/dlog-scale
+ *   [10..13] dLoss/dq of the unit quaternion (before the normalisation)
+ * each evaluated per hit in FP64 (the per-hit terms cancel strongly: a
+ * sum of products formed after FP32 accumulation would not), and
+ * k_finalize adds the colour terms and the normalisation. */
+#define GT_ACC 16
+
+struct Gauss {
   double mu[3], s[3], q[4], qn, op;
   double A[3][3]; /* A[r][c]: applied rotation (GLSL R[c][r]) */
-  double u[3], a[3], b[3], cr[3], bb, e, resp, alpha;
-  double dir[3], dist;
-  double v[3], c[3];
 };
 
-__device__ __forceinline__ void eval_hit(const gt_dev_scene& S, uint32_t id, const double* o, const double* d,
-                                         int nb, Hit& h) {
+__device__ __forceinline__ void load_gauss(const gt_dev_scene& S, uint32_t id, Gauss& g) {
   const float4* G = reinterpret_cast<const float4*>(S.geom) + 3 * (size_t)id;
   const float4 g0 = G[0], g1 = G[1], g2 = G[2];
-  h.mu[0] = g0.x; h.mu[1] = g0.y; h.mu[2] = g0.z; h.op = g0.w;
-  h.s[0] = g1.x; h.s[1] = g1.y; h.s[2] = g1.z; h.qn = g1.w;
-  h.q[0] = g2.x; h.q[1] = g2.y; h.q[2] = g2.z; h.q[3] = g2.w;
-  const double r = h.q[0], x = h.q[1], y = h.q[2], z = h.q[3];
-  h.A[0][0] = 1 - 2 * (y * y + z * z); h.A[0][1] = 2 * (x * y + r * z); h.A[0][2] = 2 * (x * z - r * y);
-  h.A[1][0] = 2 * (x * y - r * z); h.A[1][1] = 1 - 2 * (x * x + z * z); h.A[1][2] = 2 * (y * z + r * x);
-  h.A[2][0] = 2 * (x * z + r * y); h.A[2][1] = 2 * (y * z - r * x); h.A[2][2] = 1 - 2 * (x * x + y * y);
-  for (int i = 0; i < 3; ++i) h.u[i] = o[i] - h.mu[i];
+  g.mu[0] = g0.x; g.mu[1] = g0.y; g.mu[2] = g0.z; g.op = g0.w;
+  g.s[0] = g1.x; g.s[1] = g1.y; g.s[2] = g1.z; g.qn = g1.w;
+  g.q[0] = g2.x; g.q[1] = g2.y; g.q[2] = g2.z; g.q[3] = g2.w;
+  const double r = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
+  g.A[0][0] = 1 - 2 * (y * y + z * z); g.A[0][1] = 2 * (x * y + r * z); g.A[0][2] = 2 * (x * z - r * y);
+  g.A[1][0] = 2 * (x * y - r * z); g.A[1][1] = 1 - 2 * (x * x + z * z); g.A[1][2] = 2 * (y * z + r * x);
+  g.A[2][0] = 2 * (x * z + r * y); g.A[2][1] = 2 * (y * z - r * x); g.A[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+/* whitened-frame response of one hit: a, b, c = a x b, |b|^2, e, exp(-e) */
+struct Resp {
+  double a[3], b[3], cr[3], bb, e, resp;
+};
+
+__device__ __forceinline__ void eval_resp(const Gauss& g, const double* o, const double* d, Resp& h) {
+  double u[3];
+  for (int i = 0; i < 3; ++i) u[i] = o[i] - g.mu[i];
   for (int rr = 0; rr < 3; ++rr) {
-    h.a[rr] = (h.A[rr][0] * h.u[0] + h.A[rr][1] * h.u[1] + h.A[rr][2] * h.u[2]) / h.s[rr];
-    h.b[rr] = (h.A[rr][0] * d[0] + h.A[rr][1] * d[1] + h.A[rr][2] * d[2]) / h.s[rr];
+    h.a[rr] = (g.A[rr][0] * u[0] + g.A[rr][1] * u[1] + g.A[rr][2] * u[2]) / g.s[rr];
+    h.b[rr] = (g.A[rr][0] * d[0] + g.A[rr][1] * d[1] + g.A[rr][2] * d[2]) / g.s[rr];
   }
   h.cr[0] = h.a[1] * h.b[2] - h.a[2] * h.b[1];
   h.cr[1] = h.a[2] * h.b[0] - h.a[0] * h.b[2];
@@ -110,159 +129,253 @@ __device__ __forceinline__ void eval_hit(const gt_dev_scene& S, uint32_t id, con
   h.bb = h.b[0] * h.b[0] + h.b[1] * h.b[1] + h.b[2] * h.b[2];
   h.e = (h.cr[0] * h.cr[0] + h.cr[1] * h.cr[1] + h.cr[2] * h.cr[2]) / h.bb;
   h.resp = gt_exp64(-h.e);
-  h.alpha = h.resp * h.op;
-  double w[3] = {h.mu[0] - o[0], h.mu[1] - o[1], h.mu[2] - o[2]};
-  h.dist = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-  for (int i = 0; i < 3; ++i) h.dir[i] = w[i] / h.dist;
-  double bs[16], dbs[16][3];
-  sh_basis(h.dir[0], h.dir[1], h.dir[2], bs, dbs);
-  const float* sh = S.sh + 48 * (size_t)id;
-  for (int ch = 0; ch < 3; ++ch) {
-    double acc = 0.5;
-    for (int j = 0; j < nb; ++j) acc += bs[j] * (double)sh[3 * j + ch];
-    h.v[ch] = acc;
-    h.c[ch] = acc > 0 ? acc : 0.0;
+}
+
+#define BW_TABLE_LOG2 9
+#define BW_TABLE (1u << BW_TABLE_LOG2) /* LDS accumulator entries per block */
+#define BW_PROBES 8
+#define BW_EMPTY 0xFFFFFFFFu
+#define BW_VALS 14   /* accumulators in use of GT_ACC */
+
+/* Block-local accumulation: a 16x16-pixel block hits a few hundred distinct
+ * Gaussians many times each (neighbouring rays cross the same Gaussians),
+ * so the accumulators are summed in an LDS hash table keyed by Gaussian id
+ * (linear probing; keys are only ever set, never removed) and flushed to
+ * HBM once per block.  A Gaussian that finds no slot adds straight to HBM. */
+__device__ __forceinline__ int table_slot(uint32_t* key, uint32_t id) {
+  const uint32_t h = (id * 2654435761u) >> (32 - BW_TABLE_LOG2);
+  for (int pr = 0; pr < BW_PROBES; ++pr) {
+    const uint32_t e = (h + (uint32_t)pr) & (BW_TABLE - 1u);
+    const uint32_t k = __hip_atomic_load(key + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == id) return (int)e;
+    if (k == BW_EMPTY) {
+      const uint32_t old = atomicCAS(key + e, BW_EMPTY, id);
+      if (old == BW_EMPTY || old == id) return (int)e;
+    }
+  }
+  return -1;
+}
+
+/* One thread per ray; a block covers 16x16 pixels of rows [y0, y1) as four
+ * 8x8 wave tiles.  Two passes over the ray's composited ids, in FP64: the
+ * first accumulates the pixel L, the second walks the hits front to back
+ * with R_i = L - L_{<=i}. */
+__global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const gt_dev_frame F,
+                                                  const float4* __restrict__ dl_drgba,
+                                                  const float4* __restrict__ colors, uint32_t y0, uint32_t y1,
+                                                  float* __restrict__ acc, unsigned int* overflow) {
+  __shared__ uint32_t s_key[BW_TABLE];
+  __shared__ float s_val[BW_VALS * BW_TABLE];
+  for (uint32_t i = threadIdx.x; i < BW_TABLE; i += 256u) {
+    s_key[i] = BW_EMPTY;
+#pragma unroll
+    for (int k = 0; k < BW_VALS; ++k) s_val[k * BW_TABLE + i] = 0.0f;
+  }
+  __syncthreads();
+  /* accumulator k of Gaussian id: the block's LDS entry, else HBM */
+  auto add = [&](int e, uint32_t id, int k, double v) {
+    if (v == 0.0) return;
+    if (e >= 0)
+      atomicAdd(&s_val[k * BW_TABLE + (uint32_t)e], (float)v);
+    else
+      unsafeAtomicAdd(acc + (size_t)id * GT_ACC + k, (float)v);
+  };
+  const uint32_t bx_n = (F.width + 15u) / 16u;
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (blockIdx.x % bx_n) * 16u + (w & 1u) * 8u + (lane & 7u);
+  const uint32_t py = y0 + (blockIdx.x / bx_n) * 16u + (w >> 1) * 8u + (lane >> 3);
+  const uint32_t pix = py * F.width + px;
+  const uint32_t slot = pix - F.comp_base;
+  uint32_t n = 0;
+  float4 gr = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (px < F.width && py < y1) {
+    n = F.n_comp[slot];
+    if (n > F.comp_cap) {
+      if (overflow) atomicAdd(overflow, 1u);
+      n = 0;
+    }
+    if (n) gr = dl_drgba[pix];
+  }
+  const double G[3] = {gr.x, gr.y, gr.z};
+  if (n && (G[0] != 0.0 || G[1] != 0.0 || G[2] != 0.0)) {
+    const gt_f3 df = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
+    const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
+    const double d[3] = {df.x, df.y, df.z};
+    const uint32_t* ids = F.comp_ids + (size_t)slot * F.comp_cap;
+    /* pass 1: L */
+    double L[3] = {0, 0, 0}, T = 1.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t id = ids[i];
+      Gauss g;
+      load_gauss(S, id, g);
+      Resp h;
+      eval_resp(g, o, d, h);
+      const double alpha = h.resp * g.op;
+      const float4 c = colors[id];
+      L[0] += T * alpha * c.x;
+      L[1] += T * alpha * c.y;
+      L[2] += T * alpha * c.z;
+      T *= 1.0 - alpha;
+    }
+    /* pass 2: adjoints front to back */
+    double Lp[3] = {0, 0, 0};
+    T = 1.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t id = ids[i];
+      Gauss g;
+      load_gauss(S, id, g);
+      Resp h;
+      eval_resp(g, o, d, h);
+      const double alpha = h.resp * g.op;
+      const float4 c4 = colors[id];
+      const double c[3] = {c4.x, c4.y, c4.z};
+      const double one_m = 1.0 - alpha;
+      double dL_dalpha = 0.0;
+      const int e = table_slot(s_key, id);
+      for (int ch = 0; ch < 3; ++ch) {
+        Lp[ch] += T * alpha * c[ch];
+        const double rest = L[ch] - Lp[ch];
+        dL_dalpha += G[ch] * (T * c[ch] - (one_m > 0.0 ? rest / one_m : 0.0));
+        add(e, id, ch, G[ch] * T * alpha);
+      }
+      add(e, id, 3, dL_dalpha * h.resp);
+      /* response: resp = exp(-e), e = |a x b|^2 / |b|^2 */
+      const double dL_de = -h.resp * g.op * dL_dalpha;
+      const double k2 = 2.0 / h.bb;
+      const double bxc[3] = {h.b[1] * h.cr[2] - h.b[2] * h.cr[1], h.b[2] * h.cr[0] - h.b[0] * h.cr[2],
+                             h.b[0] * h.cr[1] - h.b[1] * h.cr[0]};
+      const double cxa[3] = {h.cr[1] * h.a[2] - h.cr[2] * h.a[1], h.cr[2] * h.a[0] - h.cr[0] * h.a[2],
+                             h.cr[0] * h.a[1] - h.cr[1] * h.a[0]};
+      double dL_da[3], dL_db[3];
+      for (int r = 0; r < 3; ++r) {
+        dL_da[r] = dL_de * k2 * bxc[r];
+        dL_db[r] = dL_de * k2 * (cxa[r] - h.e * h.b[r]);
+      }
+      /* a = S^-1 A (o - mu), b = S^-1 A d */
+      double M[3][3];
+      for (int r = 0; r < 3; ++r) {
+        const double is = 1.0 / g.s[r];
+        double sum = 0.0;
+        for (int cc = 0; cc < 3; ++cc) {
+          const double t = dL_da[r] * (o[cc] - g.mu[cc]) + dL_db[r] * d[cc];
+          M[r][cc] = is * t;
+          sum += g.A[r][cc] * t;
+        }
+        add(e, id, 7 + r, -is * sum); /* d(1/s)/dlog s = -1/s */
+      }
+      for (int cc = 0; cc < 3; ++cc)
+        add(e, id, 4 + cc, -(dL_da[0] * g.A[0][cc] / g.s[0] + dL_da[1] * g.A[1][cc] / g.s[1] +
+                          dL_da[2] * g.A[2][cc] / g.s[2]));
+      /* quaternion (r, x, y, z) of A */
+      const double qr = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
+      add(e, id, 10, 2 * (qz * M[0][1] - qy * M[0][2] - qz * M[1][0] + qx * M[1][2] + qy * M[2][0] - qx * M[2][1]));
+      add(e, id, 11, 2 * (qy * M[0][1] + qz * M[0][2] + qy * M[1][0] - 2 * qx * M[1][1] + qr * M[1][2] + qz * M[2][0] -
+                       qr * M[2][1] - 2 * qx * M[2][2]));
+      add(e, id, 12, 2 * (-2 * qy * M[0][0] + qx * M[0][1] - qr * M[0][2] + qx * M[1][0] + qz * M[1][2] + qr * M[2][0] +
+                       qz * M[2][1] - 2 * qy * M[2][2]));
+      add(e, id, 13, 2 * (-2 * qz * M[0][0] + qr * M[0][1] + qx * M[0][2] - qr * M[1][0] - 2 * qz * M[1][1] +
+                       qy * M[1][2] + qx * M[2][0] + qy * M[2][1]));
+      T *= one_m;
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < BW_TABLE; e += 256u) {
+    const uint32_t id = s_key[e];
+    if (id == BW_EMPTY) continue;
+    float* A = acc + (size_t)id * GT_ACC;
+#pragma unroll
+    for (int k = 0; k < BW_VALS; ++k) {
+      const float v = s_val[k * BW_TABLE + e];
+      if (v != 0.0f) unsafeAtomicAdd(A + k, v);
+    }
   }
 }
 
-__device__ __forceinline__ void add(float* p, double v) {
-  if (v != 0.0) unsafeAtomicAdd(p, (float)v);
+__global__ __launch_bounds__(256) void k_zero_f32(float* __restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) p[i] = 0.0f;
 }
 
-/* dLoss/dparams of one hit given dLoss/dalpha and dLoss/dc (per channel,
- * already including the max(0, .) clamp); accumulates into g (59 floats). */
-__device__ __forceinline__ void scatter_hit(const gt_dev_scene& S, uint32_t id, const double* o, const double* d,
-                                            int nb, const Hit& h, double dL_dalpha, const double* dL_dc, float* g) {
-  /* SH coefficients and the SH direction */
+/* One thread per Gaussian: the chain rule from the accumulators to the raw
+ * PLY parameters (mean, f_dc/f_rest, opacity logit, log-scale, raw
+ * quaternion), added into grads (n x 59). */
+__global__ __launch_bounds__(256) void k_finalize(const gt_dev_scene S, const gt_dev_frame F,
+                                                  const float* __restrict__ acc, float* __restrict__ grads) {
+  const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+  if (id >= S.n) return;
+  const float4* A4 = reinterpret_cast<const float4*>(acc + (size_t)id * GT_ACC);
+  const float4 a0 = A4[0], a1 = A4[1], a2 = A4[2], a3 = A4[3];
+  const float ac[GT_ACC] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                            a2.x, a2.y, a2.z, a2.w, a3.x, a3.y, a3.z, a3.w};
+  bool any = false;
+  for (int i = 0; i < GT_ACC; ++i) any |= ac[i] != 0.0f;
+  if (!any) return;
+  Gauss g;
+  load_gauss(S, id, g);
+  float* out = grads + (size_t)id * GT_RAW_FLOATS;
+  const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
+  const double u[3] = {o[0] - g.mu[0], o[1] - g.mu[1], o[2] - g.mu[2]};
+  /* colour: SH coefficients and the direction normalize(mu - o) */
+  const double dist = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const double dir[3] = {-u[0] / dist, -u[1] / dist, -u[2] / dist};
+  const int nb = F.sh_degree == 0 ? 1 : F.sh_degree == 1 ? 4 : F.sh_degree == 2 ? 9 : 16;
   double bs[16], dbs[16][3];
-  sh_basis(h.dir[0], h.dir[1], h.dir[2], bs, dbs);
+  sh_basis(dir[0], dir[1], dir[2], bs, dbs);
   const float* sh = S.sh + 48 * (size_t)id;
   double dL_ddir[3] = {0, 0, 0};
   for (int ch = 0; ch < 3; ++ch) {
-    if (dL_dc[ch] == 0.0) continue;
+    double v = 0.5;
+    for (int j = 0; j < nb; ++j) v += bs[j] * (double)sh[3 * j + ch];
+    const double dc = v > 0.0 ? (double)ac[ch] : 0.0;
+    if (dc == 0.0) continue;
     for (int j = 0; j < nb; ++j) {
       const int idx = j == 0 ? 3 + ch : 6 + ch * 15 + (j - 1);
-      add(g + idx, dL_dc[ch] * bs[j]);
+      out[idx] += (float)(dc * bs[j]);
       const double shv = sh[3 * j + ch];
-      for (int k = 0; k < 3; ++k) dL_ddir[k] += dL_dc[ch] * shv * dbs[j][k];
+      for (int k = 0; k < 3; ++k) dL_ddir[k] += dc * shv * dbs[j][k];
     }
   }
   double dL_dmu[3];
   {
-    const double dd = dL_ddir[0] * h.dir[0] + dL_ddir[1] * h.dir[1] + dL_ddir[2] * h.dir[2];
-    for (int k = 0; k < 3; ++k) dL_dmu[k] = (dL_ddir[k] - h.dir[k] * dd) / h.dist;
+    const double dd = dL_ddir[0] * dir[0] + dL_ddir[1] * dir[1] + dL_ddir[2] * dir[2];
+    for (int k = 0; k < 3; ++k) dL_dmu[k] = (dL_ddir[k] - dir[k] * dd) / dist;
   }
-  /* opacity logit: alpha = resp * op, op = sigmoid(logit) */
-  add(g + 51, dL_dalpha * h.resp * h.op * (1.0 - h.op));
-  /* response: resp = exp(-e), e = |a x b|^2 / |b|^2 */
-  const double dL_de = -h.resp * h.op * dL_dalpha;
-  double dL_da[3], dL_db[3];
-  {
-    const double k2 = 2.0 / h.bb;
-    const double bxc[3] = {h.b[1] * h.cr[2] - h.b[2] * h.cr[1], h.b[2] * h.cr[0] - h.b[0] * h.cr[2],
-                           h.b[0] * h.cr[1] - h.b[1] * h.cr[0]};
-    const double cxa[3] = {h.cr[1] * h.a[2] - h.cr[2] * h.a[1], h.cr[2] * h.a[0] - h.cr[0] * h.a[2],
-                           h.cr[0] * h.a[1] - h.cr[1] * h.a[0]};
-    for (int k = 0; k < 3; ++k) {
-      dL_da[k] = dL_de * k2 * bxc[k];
-      dL_db[k] = dL_de * (k2 * cxa[k] - k2 * h.e * h.b[k]);
-    }
+  /* opacity logit: alpha = response * sigmoid(logit) */
+  out[51] += (float)((double)ac[3] * g.op * (1.0 - g.op));
+  for (int k = 0; k < 3; ++k) {
+    out[k] += (float)(dL_dmu[k] + (double)ac[4 + k]);
+    out[52 + k] += ac[7 + k];
   }
-  /* a = diag(1/s) A u, b = diag(1/s) A d, u = o - mu */
-  double dL_dA[3][3], dL_dlogs[3];
-  for (int rr = 0; rr < 3; ++rr) {
-    const double is = 1.0 / h.s[rr];
-    double sum = 0.0;
-    for (int cc = 0; cc < 3; ++cc) {
-      const double t = dL_da[rr] * h.u[cc] + dL_db[rr] * d[cc];
-      dL_dA[rr][cc] = is * t;
-      sum += h.A[rr][cc] * t;
-    }
-    dL_dlogs[rr] = -is * sum; /* d(1/s)/dlog s = -1/s */
-  }
-  for (int cc = 0; cc < 3; ++cc) {
-    double du = 0.0;
-    for (int rr = 0; rr < 3; ++rr) du += dL_da[rr] * h.A[rr][cc] / h.s[rr];
-    dL_dmu[cc] -= du;
-  }
-  add(g + 0, dL_dmu[0]);
-  add(g + 1, dL_dmu[1]);
-  add(g + 2, dL_dmu[2]);
-  add(g + 52, dL_dlogs[0]);
-  add(g + 53, dL_dlogs[1]);
-  add(g + 54, dL_dlogs[2]);
-  /* quaternion (r, x, y, z) of A, then the normalisation q = raw / |raw| */
-  const double r = h.q[0], x = h.q[1], y = h.q[2], z = h.q[3];
-  const double (*M)[3] = dL_dA;
-  double dq[4];
-  dq[0] = 2 * (z * M[0][1] - y * M[0][2] - z * M[1][0] + x * M[1][2] + y * M[2][0] - x * M[2][1]);
-  dq[1] = 2 * (y * M[0][1] + z * M[0][2] + y * M[1][0] - 2 * x * M[1][1] + r * M[1][2] + z * M[2][0] - r * M[2][1] -
-               2 * x * M[2][2]);
-  dq[2] = 2 * (-2 * y * M[0][0] + x * M[0][1] - r * M[0][2] + x * M[1][0] + z * M[1][2] + r * M[2][0] + z * M[2][1] -
-               2 * y * M[2][2]);
-  dq[3] = 2 * (-2 * z * M[0][0] + r * M[0][1] + x * M[0][2] - r * M[1][0] - 2 * z * M[1][1] + y * M[1][2] +
-               x * M[2][0] + y * M[2][1]);
-  const double qd = dq[0] * r + dq[1] * x + dq[2] * y + dq[3] * z;
-  for (int k = 0; k < 4; ++k) add(g + 55 + k, (dq[k] - h.q[k] * qd) / h.qn);
+  /* the normalisation q = raw / |raw| */
+  const double dq[4] = {ac[10], ac[11], ac[12], ac[13]};
+  const double qd = dq[0] * g.q[0] + dq[1] * g.q[1] + dq[2] * g.q[2] + dq[3] * g.q[3];
+  for (int k = 0; k < 4; ++k) out[55 + k] += (float)((dq[k] - g.q[k] * qd) / g.qn);
 }
 
-__global__ __launch_bounds__(256) void k_backward(const gt_dev_scene S, const gt_dev_frame F,
-                                                  const float4* __restrict__ dl_drgba, uint32_t pix0, uint32_t npix,
-                                                  float* __restrict__ grads, unsigned int* overflow) {
-  const uint32_t li = blockIdx.x * 256u + threadIdx.x;
-  if (li >= npix) return;
-  const uint32_t pix = pix0 + li;
-  const uint32_t slot = pix - F.comp_base;
-  const uint32_t n = F.n_comp[slot];
-  if (n == 0) return;
-  if (n > F.comp_cap) {
-    if (overflow) atomicAdd(overflow, 1u);
-    return;
+__global__ void k_zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0) p[0] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ v, uint32_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) m = v[i] > m ? v[i] : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off);
+    m = o > m ? o : m;
   }
-  const float4 gr = dl_drgba[pix];
-  const double G[3] = {gr.x, gr.y, gr.z};
-  if (G[0] == 0.0 && G[1] == 0.0 && G[2] == 0.0) return;
-  const uint32_t px = pix % F.width, py = pix / F.width;
-  const gt_f3 df = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
-  const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
-  const double d[3] = {df.x, df.y, df.z};
-  const int nb = F.sh_degree == 0 ? 1 : F.sh_degree == 1 ? 4 : F.sh_degree == 2 ? 9 : 16;
-  const uint32_t* ids = F.comp_ids + (size_t)slot * F.comp_cap;
-  /* pass 1: L */
-  double L[3] = {0, 0, 0}, T = 1.0;
-  for (uint32_t i = 0; i < n; ++i) {
-    Hit h;
-    eval_hit(S, ids[i], o, d, nb, h);
-    for (int ch = 0; ch < 3; ++ch) L[ch] += T * h.alpha * h.c[ch];
-    T *= 1.0 - h.alpha;
-  }
-  /* pass 2: adjoints front to back */
-  double Lp[3] = {0, 0, 0};
-  T = 1.0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t id = ids[i];
-    Hit h;
-    eval_hit(S, id, o, d, nb, h);
-    double dL_dalpha = 0.0, dL_dc[3];
-    for (int ch = 0; ch < 3; ++ch) {
-      Lp[ch] += T * h.alpha * h.c[ch];
-      const double rest = L[ch] - Lp[ch];
-      const double one_m = 1.0 - h.alpha;
-      dL_dalpha += G[ch] * (T * h.c[ch] - (one_m > 0.0 ? rest / one_m : 0.0));
-      dL_dc[ch] = h.v[ch] > 0.0 ? G[ch] * T * h.alpha : 0.0;
-    }
-    scatter_hit(S, id, o, d, nb, h, dL_dalpha, dL_dc, grads + (size_t)id * GT_RAW_FLOATS);
-    T *= 1.0 - h.alpha;
-  }
+  if ((threadIdx.x & 63u) == 0 && m) atomicMax(out, m);
 }
 
 }  // namespace
 
-/* Backward of one frame.  All pointers are device pointers.  Checkpoint
+/* Backward of one frame.  All pointers are device pointers.  Per-Gaussian
+ * accumulators live in the scene's backward scratch, so calls on one scene
+ * must not overlap.  Checkpoint
  * mode: comp_ids / n_comp / comp_cap from a gt_render with those outputs.
- * Recompute mode (comp_ids == NULL): the forward is re-run over bands of rows
- * into a bounded scratch checkpoint (at most `budget_bytes`), each band
- * followed by its backward. */
+ * Rays that composited more than comp_cap hits are skipped and counted in
+ * *overflow.  Recompute mode (comp_ids == NULL): the forward is re-run over
+ * bands of rows into a bounded scratch checkpoint (about `budget_bytes`),
+ * each band followed by its backward; the scratch grows as needed, so no ray
+ * overflows.  Recompute mode returns after the stream has drained. */
 extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const float* dl_drgba,
                            const uint32_t* comp_ids, const uint32_t* n_comp, uint32_t comp_cap, float* grads,
                            unsigned int* overflow, uint64_t budget_bytes, void* stream) {
@@ -276,67 +389,108 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
   }
   hipStream_t st = (hipStream_t)stream;
   const gt_dev_scene* S = gt_scene_dev(s);
+  if (S->n == 0) return GT_OK;
   gt_dev_frame f;
   gt_fill_dev_frame(u, width, height, &f);
   const uint32_t R = width * height;
+  /* per-Gaussian accumulators (zeroed by a kernel, so the sequence can be
+   * captured in a graph) and this camera's colours */
+  float* acc = (float*)gt_scene_bwd_scratch(s, 0, (size_t)S->n * (GT_ACC + 4) * sizeof(float));
+  if (!acc) return GT_E_NOMEM;
+  const float4* colors = reinterpret_cast<const float4*>(acc + (size_t)S->n * GT_ACC);
+  {
+    const size_t nf = (size_t)S->n * GT_ACC;
+    const size_t blocks = (nf + 255) / 256 < 2048 ? (nf + 255) / 256 : 2048;
+    hipLaunchKernelGGL(k_zero_f32, dim3((uint32_t)blocks), dim3(256), 0, st, acc, nf);
+  }
+  if ((rc = gt_launch_colors(S, &f, acc + (size_t)S->n * GT_ACC, stream)) != GT_OK) return rc;
+  auto band_backward = [&](uint32_t y0, uint32_t y1) -> hipError_t {
+    const uint32_t blocks = ((width + 15u) / 16u) * ((y1 - y0 + 15u) / 16u);
+    hipLaunchKernelGGL(k_backward, dim3(blocks), dim3(256), 0, st, *S, f, (const float4*)dl_drgba, colors,
+                       y0, y1, acc, overflow);
+    return hipGetLastError();
+  };
+  auto finalize = [&]() -> hipError_t {
+    hipLaunchKernelGGL(k_finalize, dim3((S->n + 255u) / 256u), dim3(256), 0, st, *S, f, (const float*)acc, grads);
+    return hipGetLastError();
+  };
   if (comp_ids) {
     f.comp_ids = const_cast<uint32_t*>(comp_ids);
     f.n_comp = const_cast<uint32_t*>(n_comp);
     f.comp_cap = comp_cap;
     f.comp_base = 0;
-    hipLaunchKernelGGL(k_backward, dim3((R + 255) / 256), dim3(256), 0, st, *S, f, (const float4*)dl_drgba, 0u, R,
-                       grads, overflow);
-    const hipError_t e = hipGetLastError();
+    hipError_t e = band_backward(0, height);
+    if (e == hipSuccess) e = finalize();
     if (e != hipSuccess) { gt_set_error("backward launch failed: %s", hipGetErrorString(e)); return GT_E_HIP; }
     return GT_OK;
   }
-  /* recompute: bands of whole tile rows (8x8 tiles) */
-  const uint32_t cap = comp_cap ? comp_cap : 256u;
+  /* recompute: bands of whole tile rows (8x8 tiles).  Each band's forward
+   * writes its composited ids into the scratch checkpoint; if a ray of the
+   * band composited more than the capacity, the capacity grows (the band
+   * shrinks to stay within the budget, down to one tile row) and the band's
+   * forward runs again before its backward.  The scratch and the capacity
+   * the frame needed stay with the scene for the next call. */
+  uint32_t* last_cap = gt_scene_rc_cap(s);
+  uint32_t cap = comp_cap ? comp_cap : 64u;
+  if (*last_cap > cap) cap = *last_cap;
   gt_tuning tile8{};
   tile8.path = 1;
   tile8.tile_w = 8;
   tile8.stack_lds = gt_scene_tuning(s)->stack_lds;
-  const uint32_t tiles = gt_frame_tiles(&tile8, &f, 0);
+  (void)gt_frame_tiles(&tile8, &f, 0);
   const uint32_t tiles_y = (height + 7u) / 8u;
-  const uint64_t per_row = (uint64_t)width * 8u * (cap + 1u) * 4u;
-  const uint64_t budget = budget_bytes ? budget_bytes : (256ull << 20);
-  uint32_t band = (uint32_t)(budget / per_row);
-  if (band < 1) band = 1;
-  if (band > tiles_y) band = tiles_y;
-  const uint64_t band_px = (uint64_t)band * 8u * width;
-  void *ids_buf = nullptr, *cnt_buf = nullptr, *img = nullptr;
-  hipError_t e = hipMalloc(&ids_buf, band_px * cap * 4u);
-  if (e == hipSuccess) e = hipMalloc(&cnt_buf, band_px * 4u);
-  if (e == hipSuccess) e = hipMalloc(&img, (size_t)R * 16u);
-  if (e != hipSuccess) {
-    if (ids_buf) (void)hipFree(ids_buf);
-    if (cnt_buf) (void)hipFree(cnt_buf);
-    if (img) (void)hipFree(img);
-    gt_set_error("gt_backward: scratch allocation failed: %s", hipGetErrorString(e));
-    return GT_E_NOMEM;
-  }
-  f.rgba = (float*)img;
-  f.comp_ids = (uint32_t*)ids_buf;
-  f.n_comp = (uint32_t*)cnt_buf;
-  f.comp_cap = cap;
-  for (uint32_t ty0 = 0; ty0 < tiles_y && rc == GT_OK; ty0 += band) {
+  const uint64_t budget = budget_bytes ? budget_bytes : (1ull << 30);
+  const size_t fixed = 256 + (size_t)R * 4u + (size_t)R * 16u; /* max, counts, image */
+  char* base = nullptr;
+  hipError_t e = hipSuccess;
+  uint32_t band = 0;
+  for (uint32_t ty0 = 0; ty0 < tiles_y && rc == GT_OK && e == hipSuccess;) {
+    if (band == 0) {
+      /* (re)size the band and its checkpoint for the current capacity; the
+       * stream is idle here (first band, or after the max read-back) */
+      const uint64_t per_row = (uint64_t)width * 8u * cap * 4u;
+      band = (uint32_t)(budget / per_row);
+      if (band < 1) band = 1;
+      if (band > tiles_y) band = tiles_y;
+      base = (char*)gt_scene_bwd_scratch(s, 1, fixed + (size_t)band * 8u * width * cap * 4u);
+      if (!base) { rc = GT_E_NOMEM; break; }
+      f.rgba = (float*)(base + 256 + (size_t)R * 4u);
+      f.n_comp = (uint32_t*)(base + 256);
+      f.comp_ids = (uint32_t*)(base + fixed);
+      f.comp_cap = cap;
+    }
+    uint32_t* dmax = (uint32_t*)base;
     const uint32_t ty1 = ty0 + band < tiles_y ? ty0 + band : tiles_y;
     const uint32_t y0 = ty0 * 8u, y1 = ty1 * 8u < height ? ty1 * 8u : height;
+    const uint32_t npix = (y1 - y0) * width;
     f.comp_base = y0 * width;
     rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x);
     if (rc != GT_OK) break;
-    const uint32_t npix = (y1 - y0) * width;
-    hipLaunchKernelGGL(k_backward, dim3((npix + 255) / 256), dim3(256), 0, st, *S, f, (const float4*)dl_drgba,
-                       y0 * width, npix, grads, overflow);
-    e = hipGetLastError();
-    if (e != hipSuccess) { gt_set_error("backward launch failed: %s", hipGetErrorString(e)); rc = GT_E_HIP; }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dmax);
+    hipLaunchKernelGGL(k_max_u32, dim3((npix + 255) / 256 < 1024 ? (npix + 255) / 256 : 1024), dim3(256), 0, st,
+                       (const uint32_t*)f.n_comp, npix, dmax);
+    uint32_t mx = 0;
+    e = hipMemcpyAsync(&mx, dmax, 4u, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) break;
+    if (mx > cap) {
+      uint32_t c = 1;
+      while (c < mx) c <<= 1;
+      cap = c;
+      *last_cap = cap;
+      band = 0; /* resize, then render this band again */
+      continue;
+    }
+    e = band_backward(y0, y1);
+    if (e != hipSuccess) break;
+    ty0 = ty1;
   }
-  (void)tiles;
-  /* scratch is freed after the stream drains */
+  if (rc == GT_OK && e == hipSuccess) e = finalize();
   const hipError_t es = hipStreamSynchronize(st);
-  (void)hipFree(ids_buf);
-  (void)hipFree(cnt_buf);
-  (void)hipFree(img);
+  if (rc == GT_OK && e != hipSuccess) {
+    gt_set_error("gt_backward (recompute): %s", hipGetErrorString(e));
+    rc = GT_E_HIP;
+  }
   if (rc == GT_OK && es != hipSuccess) {
     gt_set_error("gt_backward: %s", hipGetErrorString(es));
     rc = GT_E_HIP;

@@ -396,10 +396,9 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
                 T *= 1.0f - alpha;
                 mask |= 1u << j;
                 st.h_comp++;
-                if (F.comp_ids) {
-                  if (n_comp < F.comp_cap) F.comp_ids[(size_t)(ray - F.comp_base) * F.comp_cap + n_comp] = id;
-                  ++n_comp;
-                }
+                if (F.comp_ids && n_comp < F.comp_cap)
+                  F.comp_ids[(size_t)(ray - F.comp_base) * F.comp_cap + n_comp] = id;
+                ++n_comp;
               }
             }
           }
@@ -722,6 +721,19 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
 
 __global__ void k_reset_counter(unsigned int* work) { *work = 0u; }
 
+/* Per-Gaussian radiance seen from one camera origin (the direction
+ * normalize(mu - o) of GS_shader.ts:264 depends on the Gaussian and the
+ * origin only): the same FP32 evaluation as the render kernels. */
+__global__ __launch_bounds__(256) void k_colors(const gt_dev_scene S, float ox, float oy, float oz, uint32_t deg,
+                                                float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= S.n) return;
+  const float4 g0 = reinterpret_cast<const float4*>(S.geom)[3 * (size_t)i];
+  const gt_f3 dir = gt_normalize(gt_sub(gt_f3_make(g0.x, g0.y, g0.z), gt_f3_make(ox, oy, oz)));
+  const gt_f3 c = radiance(S.sh + 48 * (size_t)i, deg, dir);
+  out[i] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
 template <int MAXK, int MODE, bool BATCH>
 hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                         uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
@@ -797,6 +809,18 @@ hipError_t ensure_faces() {
 static void tuning_defaults(const gt_tuning* t, uint32_t* tile_w, uint32_t* lds_depth) {
   *tile_w = (t && t->tile_w) ? t->tile_w : 8u;
   *lds_depth = (t && t->stack_lds) ? t->stack_lds : (uint32_t)GT_LDS_STACK;
+}
+
+extern "C" int gt_launch_colors(const gt_dev_scene* S, const gt_dev_frame* F, float* out, void* stream) {
+  if (S->n == 0) return GT_OK;
+  hipLaunchKernelGGL(k_colors, dim3((S->n + 255u) / 256u), dim3(256), 0, (hipStream_t)stream, *S, F->to_world[12],
+                     F->to_world[13], F->to_world[14], F->sh_degree, reinterpret_cast<float4*>(out));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    gt_set_error("colour launch failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  return GT_OK;
 }
 
 extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, const gt_dev_frame* cams_dev,

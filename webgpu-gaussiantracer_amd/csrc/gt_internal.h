@@ -90,6 +90,9 @@ int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev
 int gt_launch_persistent(const gt_dev_scene* s, const gt_dev_frame* cam0, const gt_dev_frame* cams_dev,
                          uint32_t n_cams, uint32_t total_tiles, unsigned int* work, unsigned long long* stats,
                          const gt_tuning* tune, void* stream);
+/* out[i] (float4, N entries) = FP32 radiance of Gaussian i seen from f's
+ * camera origin, bit-identical to the colour the render kernels composite. */
+int gt_launch_colors(const gt_dev_scene* s, const gt_dev_frame* f, float* out, void* stream);
 void gt_set_error(const char* fmt, ...);
 #ifdef __cplusplus
 }

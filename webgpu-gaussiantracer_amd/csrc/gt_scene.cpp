@@ -54,6 +54,9 @@ struct gt_scene {
   int device = 0;
   DevBuf geom, sh, prox, nodes, faces;
   DevBuf ctl; /* work counter of the persistent renderer */
+  DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
+  DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
+  uint32_t rc_cap = 0; /* checkpoint capacity the last recompute needed */
   gt_tuning tune{};
   gt_dev_scene dev{};
   uint32_t n_nodes = 0, max_stack = 0, depth = 0;
@@ -669,7 +672,7 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
 static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   const uint32_t tiles = gt_frame_tiles(&s->tune, f, 0);
   /* the checkpoint for the backward pass is written by the tile kernel */
-  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids)
+  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids || f->n_comp)
     return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream, 0);
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
                               &s->tune, stream);
@@ -699,6 +702,28 @@ const gt_dev_scene* gt_scene_dev(const gt_scene* s) { return &s->dev; }
 const gt_tuning* gt_scene_tuning(const gt_scene* s) { return &s->tune; }
 unsigned int* gt_scene_work(gt_scene* s) { return (unsigned int*)s->ctl.p; }
 
+static void* grow(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return b.p;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  const hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    gt_set_error("backward scratch hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.p;
+}
+
+/* Backward scratch of at least `bytes` (contents not kept when it grows),
+ * allocated on first use and kept with the scene; NULL (with the error set)
+ * when the allocation fails.  `which` 0: accumulators and colours, 1: the
+ * recompute mode's band buffers. */
+void* gt_scene_bwd_scratch(gt_scene* s, int which, size_t bytes) { return grow(which ? s->rc : s->bwd, bytes); }
+uint32_t* gt_scene_rc_cap(gt_scene* s) { return &s->rc_cap; }
+
 extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const gt_frame* out,
                          int buf_flags, void* stream) {
   if (!s || !out || !out->rgba) { gt_set_error("gt_render: scene, out and out->rgba are required"); return GT_E_INVALID; }
@@ -708,11 +733,11 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     gt_set_error("gt_render: max_rounds must be >= 1 when hit_ids/accept are requested");
     return GT_E_INVALID;
   }
-  if ((out->comp_ids != nullptr) != (out->n_comp != nullptr) || (out->comp_ids && out->comp_cap == 0)) {
+  if (out->comp_ids && (!out->n_comp || out->comp_cap == 0)) {
     gt_set_error("gt_render: the checkpoint needs comp_ids, n_comp and comp_cap >= 1");
     return GT_E_INVALID;
   }
-  if (out->comp_ids && u->debug_mode != 0) {
+  if ((out->comp_ids || out->n_comp) && u->debug_mode != 0) {
     gt_set_error("gt_render: the checkpoint is written in debugMode 0 only");
     return GT_E_INVALID;
   }
@@ -772,10 +797,12 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
   }
   if (out->comp_ids) {
     if ((rc = alloc(t_cids, R * out->comp_cap * 4)) != GT_OK) return rc;
-    if ((rc = alloc(t_ncomp, R * 4)) != GT_OK) return rc;
     f.comp_ids = (uint32_t*)t_cids.p;
-    f.n_comp = (uint32_t*)t_ncomp.p;
     f.comp_cap = out->comp_cap;
+  }
+  if (out->n_comp) {
+    if ((rc = alloc(t_ncomp, R * 4)) != GT_OK) return rc;
+    f.n_comp = (uint32_t*)t_ncomp.p;
   }
   f.max_rounds = (uint32_t)MR;
   if ((rc = launch_frame(s, &f, nullptr)) != GT_OK) return rc;
@@ -786,10 +813,8 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
   if (out->n_rounds) HIP_TRY(hipMemcpy(out->n_rounds, f.n_rounds, R * 4, hipMemcpyDeviceToHost));
   if (out->T) HIP_TRY(hipMemcpy(out->T, f.T_out, R * 4, hipMemcpyDeviceToHost));
   if (out->stats) HIP_TRY(hipMemcpy(out->stats, f.stats, GT_NSTATS * 8, hipMemcpyDeviceToHost));
-  if (out->comp_ids) {
-    HIP_TRY(hipMemcpy(out->comp_ids, f.comp_ids, R * out->comp_cap * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(out->n_comp, f.n_comp, R * 4, hipMemcpyDeviceToHost));
-  }
+  if (out->comp_ids) HIP_TRY(hipMemcpy(out->comp_ids, f.comp_ids, R * out->comp_cap * 4, hipMemcpyDeviceToHost));
+  if (out->n_comp) HIP_TRY(hipMemcpy(out->n_comp, f.n_comp, R * 4, hipMemcpyDeviceToHost));
   return GT_OK;
 }
 

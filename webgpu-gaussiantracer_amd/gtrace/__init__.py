@@ -311,7 +311,8 @@ class Scene:
     def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False, checkpoint_cap=0):
         """traceRays(W, H) with host buffers: returns rgba (H, W, 4) and, on request,
         hit lists (H*W, max_rounds, k), accept masks, round counts, T, stats and
-        the backward checkpoint (composited ids per ray, capacity checkpoint_cap)."""
+        the backward checkpoint (composited ids per ray, capacity checkpoint_cap;
+        checkpoint_cap=-1 returns only the per-ray counts)."""
         R = width * height
         rgba = np.zeros((height, width, 4), np.float32)
         out = {"rgba": rgba}
@@ -333,10 +334,13 @@ class Scene:
             f.stats = _ptr(st)
             out["stats"] = st
         if checkpoint_cap:
-            cids = np.zeros((R, checkpoint_cap), np.uint32)
             ncomp = np.zeros(R, np.uint32)
-            f.comp_ids, f.n_comp, f.comp_cap = _ptr(cids), _ptr(ncomp), checkpoint_cap
-            out.update(comp_ids=cids, n_comp=ncomp)
+            f.n_comp = _ptr(ncomp)
+            out.update(n_comp=ncomp)
+            if checkpoint_cap > 0:
+                cids = np.zeros((R, checkpoint_cap), np.uint32)
+                f.comp_ids, f.comp_cap = _ptr(cids), checkpoint_cap
+                out.update(comp_ids=cids)
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 0, None))
         if want_stats:
             out["stats"] = dict(zip(STAT_NAMES, [int(x) for x in out["stats"]]))

@@ -12,7 +12,7 @@ import torch
 
 from . import SENTINEL, Scene, backward as _backward
 
-CHECKPOINT_CAP = 256
+CHECKPOINT_CAP = 256  # initial composited ids stored per ray; grows when a ray composites more
 
 
 class RenderFunction(torch.autograd.Function):
@@ -23,11 +23,17 @@ class RenderFunction(torch.autograd.Function):
         rgba = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
         stream = torch.cuda.current_stream().cuda_stream
         if mode == "checkpoint":
-            cids = torch.empty((height * width, CHECKPOINT_CAP), dtype=torch.int32, device=dev)
+            cap = CHECKPOINT_CAP
             ncomp = torch.empty(height * width, dtype=torch.int32, device=dev)
-            scene.render_device(u, width, height, rgba.data_ptr(), stream, comp_ids_ptr=cids.data_ptr(),
-                                n_comp_ptr=ncomp.data_ptr(), comp_cap=CHECKPOINT_CAP)
-            ctx.ckpt = (cids, ncomp)
+            while True:
+                cids = torch.empty((height * width, cap), dtype=torch.int32, device=dev)
+                scene.render_device(u, width, height, rgba.data_ptr(), stream, comp_ids_ptr=cids.data_ptr(),
+                                    n_comp_ptr=ncomp.data_ptr(), comp_cap=cap)
+                most = int(ncomp.max().item()) if ncomp.numel() else 0
+                if most <= cap:
+                    break
+                cap = 1 << (most - 1).bit_length()  # a ray composited more: render again with room for it
+            ctx.ckpt = (cids, ncomp, cap)
         elif mode == "recompute":
             scene.render_device(u, width, height, rgba.data_ptr(), stream)
             ctx.ckpt = None
@@ -46,14 +52,14 @@ class RenderFunction(torch.autograd.Function):
         overflow = torch.zeros(1, dtype=torch.int32, device=dev)
         stream = torch.cuda.current_stream().cuda_stream
         if ctx.ckpt is not None:
-            cids, ncomp = ctx.ckpt
+            cids, ncomp, cap = ctx.ckpt
             _backward(ctx.scene, ctx.u, width, height, g.data_ptr(), grads.data_ptr(), cids.data_ptr(),
-                      ncomp.data_ptr(), CHECKPOINT_CAP, overflow.data_ptr(), 0, stream)
+                      ncomp.data_ptr(), cap, overflow.data_ptr(), 0, stream)
         else:
             _backward(ctx.scene, ctx.u, width, height, g.data_ptr(), grads.data_ptr(), None, None,
                       CHECKPOINT_CAP, overflow.data_ptr(), 0, stream)
         if int(overflow.item()) != 0:
-            raise RuntimeError(f"{int(overflow.item())} rays composited more than {CHECKPOINT_CAP} hits")
+            raise RuntimeError(f"{int(overflow.item())} rays overflowed the backward checkpoint")
         return grads.to(ctx.raw_device, ctx.raw_dtype), None, None, None, None, None
 
 
