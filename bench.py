@@ -34,6 +34,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "webgpu-gaussiantracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_forward launch of the default frame (1M Gaussians,
+# 1920x1080, k 16), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# this script scaled by the counters' reading for a 1 GiB device copy
+# (x2.000 and x1.000): profiles/k_forward_1M_1080p_k16.md.  Reported as
+# roofline.traffic for that workload only.
+TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 1500693900, "write_bytes": 474520272}
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
 
 
@@ -334,7 +340,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": (TRAFFIC_DEFAULT_FRAME["fetch_bytes"] + TRAFFIC_DEFAULT_FRAME["write_bytes"]
+                        if (args.n, W, H, K) == (1000000, 1920, 1080, 16) else None),
             "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16,0,false,false>",
         },
         "stats": st,
@@ -344,6 +352,17 @@ def main():
                   "host_build_s": round(t_build, 3), "generate_s": round(t_gen, 3)},
         "image_mean": float(img[..., :3].mean()),
     }
+
+    # host-buffer path (what the Node addon uses): device temporaries, render,
+    # copy-back over PCIe, synchronous -- reported beside value, never as it
+    hp = []
+    for _ in range(3):
+        h0 = time.perf_counter()
+        scene.render(u, W, H)
+        hp.append(time.perf_counter() - h0)
+    hp_s = sorted(hp)[1]
+    out["host_path"] = {"ms_per_frame": round(hp_s * 1e3, 3), "mrays": round(R / hp_s / 1e6, 3),
+                        "note": "gt_render with host buffers (allocation, render, PCIe copy-back), median of 3"}
 
     if not args.no_backward:
         out["fwd_bwd"] = bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, max(2, args.steps // 2),

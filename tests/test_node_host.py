@@ -1,0 +1,80 @@
+"""The JavaScript host (webgpu-gaussiantracer_amd/node) over the Node-API
+addon: host-only entry points compared with the Python binding of the same
+library on the CPU; a render through the addon compared bit for bit with the
+Python binding's render on the GPU."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import gtrace as gt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE_DIR = os.path.join(ROOT, "webgpu-gaussiantracer_amd", "node")
+ADDON = os.path.join(ROOT, "webgpu-gaussiantracer_amd", "lib", "gtrace.node")
+
+pytestmark = pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is absent")
+
+
+def _node(script, tmp_path):
+    f = tmp_path / "s.js"
+    f.write_text(f"const gt = require({json.dumps(NODE_DIR)});\nconst fs = require('fs');\n" + script)
+    r = subprocess.run(["node", str(f)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_host_entry_points_match_python_binding(tmp_path):
+    out = tmp_path / "o.bin"
+    ply = os.path.join(ROOT, "tests", "golden", "pc_short.ply")
+    _node(f"""
+const c = new gt.WASDCamera({{position: [1.5, -2, 7], target: [0.25, 0, 0]}});
+const raw = gt.generateScene({{n: 257, nClusters: 4, boxHalf: 3, seed: 11}});
+const p = gt.loadPly({json.dumps(ply)});
+let err = '';
+try {{ gt.loadPly('/nonexistent/x.ply'); }} catch (e) {{ err = e.message; }}
+if (!err.startsWith('gt error')) throw new Error('missing error: ' + err);
+const parts = [c.matrix, raw, p.raw].map(a => Buffer.from(a.buffer, a.byteOffset, a.byteLength));
+fs.writeFileSync({json.dumps(str(out))}, Buffer.concat(parts));
+console.log(JSON.stringify({{n: p.n, deg: p.shDegree, version: gt.version()}}));
+""", tmp_path)
+    blob = np.fromfile(out, np.float32)
+    m, rest = blob[:16], blob[16:]
+    g, pl = rest[:257 * 59], rest[257 * 59:]
+    np.testing.assert_array_equal(m, gt.camera_look_at([1.5, -2, 7], [0.25, 0, 0]))
+    np.testing.assert_array_equal(g.reshape(257, 59), gt.generate_raw(n=257, n_clusters=4, box_half=3.0, seed=11))
+    ref = gt.read_ply(os.path.join(ROOT, "tests", "golden", "pc_short.ply"))
+    np.testing.assert_array_equal(pl.reshape(-1, 59), ref[0])
+
+
+@pytest.mark.gpu
+def test_render_through_addon_equals_python_binding(tmp_path):
+    print("case node addon render 64x48 c1 k6", flush=True)
+    out = tmp_path / "rgba.bin"
+    _node(f"""
+const raw = gt.generateScene({{n: 4000, boxHalf: 2, logScaleMean: -3, logScaleStd: 0.5, seed: 0}});
+const tracer = new gt.GaussianTracer(raw, 4000, 0.1);
+const cam = new gt.WASDCamera({{position: [0, 0, 8], target: [0, 0, 0]}});
+const a = tracer.render(cam, {{alpha_min: 0.1, hit_array_size: 6, T_min: 0.03, debugMode: 0, width: 64, height: 48}});
+const b = tracer.render(cam, {{alpha_min: 0.2, hit_array_size: 6, T_min: 0.03, debugMode: 0, width: 64, height: 48}});
+const info = tracer.info();
+tracer.destroy();
+let err = '';
+try {{ tracer.render(cam, {{width: 8, height: 8}}); }} catch (e) {{ err = e.message; }}
+if (!err) throw new Error('render after destroy did not throw');
+fs.writeFileSync({json.dumps(str(out))}, Buffer.concat([a, b].map(x => Buffer.from(x.buffer))));
+console.log(JSON.stringify(info));
+""", tmp_path)
+    blob = np.fromfile(out, np.float32).reshape(2, 48, 64, 4)
+    raw = gt.generate_raw(n=4000, box_half=2.0, log_scale_mean=-3.0, log_scale_std=0.5, seed=0)
+    tw = gt.camera_look_at([0, 0, 8], [0, 0, 0])
+    for i, amin in enumerate((0.1, 0.2)):
+        s = gt.Scene(raw, alpha_min=amin)
+        u = gt.make_uniforms(tw, 64, 48, hfov_deg=60, alpha_min=amin, k=6, T_min=0.03)
+        want = s.render(u, 64, 48)["rgba"].reshape(48, 64, 4)
+        np.testing.assert_array_equal(blob[i], want)
+    assert not np.array_equal(blob[0], blob[1])

@@ -61,6 +61,24 @@ def build_lib(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs +
          ["-Wl,-soname,libgt.so"], verbose)
     os.replace(out + ".tmp", out)
+    build_node_addon(verbose)
+    return out
+
+
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_node_addon(verbose=False):
+    """lib/gtrace.node: the Node-API addon of the JavaScript host (node/),
+    linked against lib/libgt.so (found next to it at run time).  Skipped
+    when the Node headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    out = os.path.join(LIBDIR, "gtrace.node")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + NODE_INCLUDE,
+          os.path.join(PKG, "node", "gt_napi.cpp"), "-o", out + ".tmp", "-L" + LIBDIR, "-lgt",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(out + ".tmp", out)
     return out
 
 

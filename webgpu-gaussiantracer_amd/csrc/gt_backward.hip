@@ -322,12 +322,17 @@ __global__ __launch_bounds__(256) void k_finalize(const gt_dev_scene S, const gt
   sh_basis(dir[0], dir[1], dir[2], bs, dbs);
   const float* sh = S.sh + 48 * (size_t)id;
   double dL_ddir[3] = {0, 0, 0};
+#pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
     double v = 0.5;
-    for (int j = 0; j < nb; ++j) v += bs[j] * (double)sh[3 * j + ch];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < nb) v += bs[j] * (double)sh[3 * j + ch];
     const double dc = v > 0.0 ? (double)ac[ch] : 0.0;
     if (dc == 0.0) continue;
-    for (int j = 0; j < nb; ++j) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j >= nb) continue;
       const int idx = j == 0 ? 3 + ch : 6 + ch * 15 + (j - 1);
       out[idx] += (float)(dc * bs[j]);
       const double shv = sh[3 * j + ch];
