@@ -20,6 +20,8 @@ fwd_bwd: forward with the backward checkpoint + dL/dRGBA of a squared-error
 loss + the hand-written backward, on the same frame (and the recompute mode).
 batched: 64 cameras of uneven resolution in one launch replayed from a
 hipGraph, against the same cameras rendered one launch at a time.
+c5: 3M Gaussians, 16 views at 1280x720, forward + recompute-mode backward
+per view with gradients accumulated across views, peak device memory.
 
 cpu_baseline: the oracle (the project's C restatement of the reference path,
 BVH variant, all host threads up to 16) on a seeded 1/3 sample of the rays.
@@ -236,6 +238,70 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
     }
 
 
+def bench_c5(gt, torch, dev, sp, steps):
+    """Batched forward + backward at scale: 3M Gaussians (256-cluster mixture in
+    a 40^3 box, log-scales N(-4.5, 0.7)), 16 cameras at 1280x720 on a sphere of
+    radius 25 (seed 1); per view: forward, dL/dRGBA of sum (render - target)^2
+    (target from means + N(0, 0.01), seed 3), recompute-mode backward
+    accumulating into one 3M x 59 gradient with FP32 atomics.  Reports the
+    peak device memory in use during the timed steps."""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    raw = gt.generate_raw(seed=0, config="c5")
+    n = raw.shape[0]
+    scene = gt.Scene(raw, alpha_min=0.01)
+    rng = np.random.Generator(np.random.PCG64(3))
+    raw_t = raw.copy()
+    raw_t[:, 0:3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    W, H, V = 1280, 720, 16
+    rngc = np.random.Generator(np.random.PCG64(1))
+    dirs = rngc.normal(size=(V, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    us = [gt.make_uniforms(gt.camera_look_at(list(25.0 * d), [0.0, 0.0, 0.0]), W, H, hfov_deg=50.0, T_min=1e-3,
+                           alpha_min=0.01, k=16) for d in dirs]
+    targets = torch.empty((V, H, W, 4), dtype=torch.float32, device=dev)
+    ts = gt.Scene(raw_t, alpha_min=0.01)
+    for i, u in enumerate(us):
+        ts.render_device(u, W, H, targets[i].data_ptr(), sp)
+    torch.cuda.synchronize()
+    ts.close()
+    del raw_t
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    gimg = torch.empty_like(rgba)
+    grads = torch.zeros((n, gt.GT_RAW_FLOATS), dtype=torch.float32, device=dev)
+    ovf = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def fwd():
+        for u in us:
+            scene.render_device(u, W, H, rgba.data_ptr(), sp)
+
+    def step():
+        grads.zero_()
+        for i, u in enumerate(us):
+            scene.render_device(u, W, H, rgba.data_ptr(), sp)
+            torch.sub(rgba, targets[i], out=gimg)
+            gimg.mul_(2.0)
+            gt.backward(scene, u, W, H, gimg.data_ptr(), grads.data_ptr(), None, None, 0, ovf.data_ptr(), 0, sp)
+
+    step()
+    torch.cuda.synchronize()
+    with MemSampler(torch) as mem:
+        ms = _events_ms(torch, stream, step, steps)
+    ms_f = _events_ms(torch, stream, fwd, steps)
+    assert int(ovf.item()) == 0
+    rays = V * W * H
+    info = scene.info()
+    return {
+        "workload": f"{n} Gaussians 256-cluster mixture, {V} cameras {W}x{H} on a sphere of radius 25, alpha_min "
+                    f"0.01, k 16, T_min 0.001; per view forward + dL/dRGBA + recompute-mode backward, gradients "
+                    f"accumulated across views",
+        "rays_per_step": rays, "ms_per_step": round(ms, 3), "mrays": round(rays / ms / 1e3, 3),
+        "ms_forward_only": round(ms_f, 3), "ratio_vs_forward": round(ms / ms_f, 3),
+        "peak_device_bytes": int(mem.peak), "scene_device_bytes": info["device_bytes"],
+        "gradient_bytes": n * gt.GT_RAW_FLOATS * 4, "grad_norm": float(grads.norm().item()), "steps": steps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +315,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-camera measurement")
     ap.add_argument("--batch-cams", type=int, default=64)
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 3M-Gaussian 16-view forward+backward")
     ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -369,6 +436,8 @@ def main():
                                        min(args.warmup, 2), ms_step, st)
     if not args.no_batched:
         out["batched"] = bench_batched(gt, scene, torch, dev, sp, K, args.batch_cams)
+    if not args.no_c5:
+        out["c5"] = bench_c5(gt, torch, dev, sp, 2)
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -228,3 +228,93 @@ def test_count_only_checkpoint():
     cnt = s.render(u, 48, 40, checkpoint_cap=-1)
     np.testing.assert_array_equal(cnt["n_comp"], full["n_comp"])
     np.testing.assert_array_equal(cnt["rgba"], full["rgba"])
+
+
+C4_CAMERA = ([0.0, 0.5, 4.0], [0.0, 0.0, 0.0])
+
+
+def test_c4_scale_fd_512_and_modes():
+    """200k single-cluster Gaussians, one 800x800 camera, alpha_min 0.01, k 16,
+    T_min 0.001; loss = sum (render - target)^2 with the target rendered from
+    means + N(0, 0.01).  Kernel gradients vs FP64 central differences (step
+    1e-4) of the FP64 re-render of the frozen hit lists on 512 seeded
+    parameters (seed 2), and checkpoint vs recompute mode."""
+    _print("case C4 200k 800x800 k16: FD 512 + checkpoint vs recompute")
+    W = H = 800
+    raw = gt.generate_raw(seed=0, config="c4")
+    n = raw.shape[0]
+    s = gt.Scene(raw, alpha_min=0.01)
+    u = uniforms(C4_CAMERA[0], C4_CAMERA[1], W, H, hfov_deg=50, alpha_min=0.01, k=16, T_min=1e-3)
+    rng = np.random.default_rng(3)
+    raw_t = raw.copy()
+    raw_t[:, 0:3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    target = gt.Scene(raw_t, alpha_min=0.01).render(u, W, H)["rgba"].reshape(H * W, 4)
+    _print(f"  C4: target rendered")
+    cnt = s.render(u, W, H, checkpoint_cap=-1)["n_comp"]
+    cap = 1 << int(max(1, cnt.max()) - 1).bit_length()
+    ck = s.render(u, W, H, checkpoint_cap=cap)
+    R = W * H
+    rays, gids, pos = fr.flatten_checkpoint(ck["comp_ids"], ck["n_comp"])
+    p = orc.params_from_uniforms(u, W, H)
+    dirs_all = np.stack([orc.ray_dir(p, r % W, r // W) for r in range(R)]).astype(np.float64)
+    o = torch.tensor([u.to_world[12], u.to_world[13], u.to_world[14]], dtype=torch.float64)
+    raw64 = torch.from_numpy(raw.astype(np.float64))
+    tgt64 = torch.from_numpy(target[:, :3].astype(np.float64))
+    # FP64 pixels of the frozen hit lists, in chunks of rays; the loss
+    # gradient dL/dRGBA = 2 (render64 - target) is handed to the kernel
+    out64 = np.zeros((R, 3))
+    bounds = list(range(0, R, 65536)) + [R]
+    starts = np.searchsorted(rays, bounds)
+    with torch.no_grad():
+        for c in range(len(bounds) - 1):
+            a, b = starts[c], starts[c + 1]
+            r0 = bounds[c]
+            nr = bounds[c + 1] - r0
+            out64[r0:r0 + nr] = fr.render(raw64, o, torch.from_numpy(dirs_all[r0:r0 + nr]), rays[a:b] - r0,
+                                          gids[a:b], pos[a:b], nr).numpy()
+    assert np.abs(out64 - ck["rgba"].reshape(R, 4)[:, :3]).max() <= 1e-4
+    G = np.zeros((R, 4), np.float32)
+    G[:, :3] = 2.0 * (out64 - target[:, :3].astype(np.float64))
+    G = G.reshape(H, W, 4)
+    g_ck, ovf = _gpu_grads(s, n, u, W, H, G, "checkpoint", cap=cap)
+    assert ovf == 0
+    g_rc, ovf = _gpu_grads(s, n, u, W, H, G, "recompute")
+    assert ovf == 0
+    np.testing.assert_allclose(g_rc, g_ck, rtol=1e-5, atol=1e-6 * np.abs(g_ck).max())
+    # FD on the rays that composite each picked Gaussian
+    order = np.argsort(gids, kind="stable")
+    gs, rs = gids[order], rays[order]
+    hit = np.unique(gids)
+    pick_rng = np.random.default_rng(2)
+    per = {"mean": 96, "f_dc": 64, "f_rest": 96, "logit": 64, "log_scale": 96, "quat": 96}
+    picks = []
+    for name, cnt_ in per.items():
+        sl = GROUPS[name]
+        picks += [(int(pick_rng.choice(hit)), int(pick_rng.integers(sl.start, sl.stop))) for _ in range(cnt_)]
+    h = 1e-4
+    fd = np.zeros(len(picks))
+    with torch.no_grad():
+        for i, (gid, col) in enumerate(picks):
+            lo, hi = np.searchsorted(gs, gid), np.searchsorted(gs, gid, side="right")
+            sub = np.unique(rs[lo:hi])
+            srays, sgids, spos = fr.flatten_checkpoint(ck["comp_ids"][sub], ck["n_comp"][sub])
+            dirs = torch.from_numpy(dirs_all[sub])
+            tsub = tgt64[torch.from_numpy(sub)]
+
+            def loss():
+                out = fr.render(raw64, o, dirs, srays, sgids, spos, len(sub))
+                return float(((out - tsub) ** 2).sum())
+
+            raw64[gid, col] += h
+            lp = loss()
+            raw64[gid, col] -= 2 * h
+            lm = loss()
+            raw64[gid, col] += h
+            fd[i] = (lp - lm) / (2 * h)
+    gv = np.array([g_ck[a, b] for a, b in picks])
+    scale = np.abs(fd).max()
+    err = np.abs(gv - fd)
+    tol = 1e-3 * (np.abs(fd) + 1e-2 * scale)
+    _print(f"  C4 FD: max |kernel - fd| {err.max():.3e}, max |fd| {scale:.3e}, worst ratio {(err / tol).max():.3f}")
+    bad = np.nonzero(err > tol)[0]
+    assert bad.size == 0, f"{bad.size} of 512 differ, first {[picks[i] for i in bad[:4]]} {gv[bad[:4]]} {fd[bad[:4]]}"
