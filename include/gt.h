@@ -132,8 +132,10 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
 /* Launch tuning of a scene (no counterpart in the reference; chosen per
  * scene by the autotuner).  Zero fields mean "library default".  None of
  * them changes a result: hit lists and pixels are identical for all values.
- *   path          0 default, 1 one 8x8-tile block per wave, 2 persistent
- *                 kernel with a global ray queue
+ *   path          0 default (= 3), 1 tile kernel (one ray tile per wave,
+ *                 per-lane traversal), 2 persistent kernel with a global ray
+ *                 queue, 3 tile kernel with wave-wide packet traversal (the
+ *                 wave walks the BVH together from one stack of (node, lane mask))
  *   refill_min    persistent: idle lanes of a wave that trigger a refill (1..64)
  *   comp_min      persistent: lanes waiting to composite that trigger a
  *                 composite pass (1..64)
@@ -236,7 +238,7 @@ void gt_batch_destroy(gt_batch* b);
 typedef struct gt_autotune_result {
   gt_tuning best;
   double best_ms;   /* per render, best candidate */
-  double first_ms;  /* per render, first candidate (tile kernel, 8x8, 16-entry LDS stack) */
+  double first_ms;  /* per render, first candidate (the default: packet traversal, 8x8 tiles) */
   uint32_t n_tried;
   uint32_t _pad;
 } gt_autotune_result;

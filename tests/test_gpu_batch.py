@@ -75,7 +75,7 @@ def _same(batch_out, single):
         assert b["rgba"].tobytes() == r["rgba"].tobytes(), f"camera {i}: pixels"
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path", [1, 2, 3])
 def test_batch_equals_single_camera(path):
     print(f"case batch path {path}", flush=True)
     s, c = _scene()
@@ -86,7 +86,7 @@ def test_batch_equals_single_camera(path):
     _same(out, single)
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path", [1, 2, 3])
 def test_batch_graph_replays(path):
     print(f"case batch graph path {path}", flush=True)
     s, c = _scene()
@@ -100,7 +100,8 @@ def test_batch_graph_replays(path):
 
 TUNINGS = [dict(path=1, tile_w=8), dict(path=1, tile_w=16), dict(path=1, tile_w=32, stack_lds=8),
            dict(path=1, tile_w=4, stack_lds=64), dict(path=2, refill_min=1, comp_min=1),
-           dict(path=2, refill_min=64, comp_min=64, tile_w=16, stack_lds=16), dict(path=2, blocks_per_cu=1)]
+           dict(path=2, refill_min=64, comp_min=64, tile_w=16, stack_lds=16), dict(path=2, blocks_per_cu=1),
+           dict(path=3, tile_w=8), dict(path=3, tile_w=4), dict(path=3, tile_w=32)]
 
 
 def test_tuning_never_changes_results():

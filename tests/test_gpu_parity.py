@@ -144,3 +144,36 @@ def test_determinism_same_frame_twice():
     a = s.render(u, 96, 64, want_T=True)
     b = s.render(u, 96, 64, want_T=True)
     assert a["rgba"].tobytes() == b["rgba"].tobytes() and a["T"].tobytes() == b["T"].tobytes()
+
+
+@pytest.mark.parametrize("k", [1, 6, 16, 32])
+def test_packet_traversal_parity(k):
+    """Wave-wide packet traversal (tuning path 3) against the oracle: the
+    clustered scene with the camera inside, and the C1 scene."""
+    _print(f"case packet traversal k{k}")
+    raw = gt.generate_raw(n=100000, seed=0, config="c2", n_clusters=8)
+    s, o = scene_pair(raw, 0.01)
+    s.set_tuning(path=3)
+    c0 = raw[0::8, :3].mean(axis=0)
+    u = uniforms(list(c0 + np.array([0.3, 0.2, 1.5])), list(c0), 96, 64, hfov_deg=50, alpha_min=0.01, k=k,
+                 T_min=1e-3)
+    check_parity(s, o, u, 96, 64, max_rounds=max(16, 512 // k), label=f"packet clusters k{k}")
+    raw = gt.generate_raw(seed=0, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    s.set_tuning(path=3, tile_w=16)
+    u = uniforms([0, 0, 8], [0, 0, 0], 80, 72, hfov_deg=60, alpha_min=0.1, k=k, T_min=0.03)
+    check_parity(s, o, u, 80, 72, max_rounds=max(16, 512 // k), label=f"packet c1 k{k}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_packet_traversal_debug_modes(mode):
+    _print(f"case packet debug mode {mode}")
+    raw = gt.generate_raw(n=20000, seed=7, config="c1")
+    s, o = scene_pair(raw, 0.1)
+    s.set_tuning(path=3)
+    u = uniforms([0, 0, 8], [0, 0, 0], 64, 64, alpha_min=0.1, k=6, debug_mode=mode)
+    g = s.render(u, 64, 64, max_rounds=1)
+    import oracle as orc
+    r = orc.render(o, orc.params_from_uniforms(u, 64, 64), max_rounds=1)
+    np.testing.assert_array_equal(g["hit_ids"], r["hit_ids"])
+    np.testing.assert_allclose(g["rgba"].reshape(-1, 4), r["rgba"], atol=1e-5, rtol=0)

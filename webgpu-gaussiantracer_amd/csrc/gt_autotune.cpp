@@ -1,8 +1,9 @@
 /*
  * gt_autotune.cpp - per-scene choice of the launch tuning by measurement.
  *
- * Candidates differ only in how work is scheduled (tile kernel or
- * persistent ray queue, ray-tile shape of a wave, LDS stack depth,
+ * Candidates differ only in how work is scheduled (tile kernel with
+ * per-lane or wave-wide packet traversal, persistent ray queue, ray-tile
+ * shape of a wave, LDS stack depth,
  * persistent refill / composite thresholds); none changes a result.  Each
  * candidate renders the caller's workload (one frame or one batch) once to
  * warm up and then `iters` times between HIP events on the caller's stream;
@@ -19,6 +20,7 @@ namespace {
 
 std::vector<gt_tuning> candidates() {
   std::vector<gt_tuning> c;
+  for (uint32_t tw : {8u, 4u, 16u}) c.push_back(gt_tuning{3, 0, 0, 0, tw, 0});
   for (uint32_t tw : {8u, 16u, 32u})
     for (uint32_t lds : {16u, 32u}) c.push_back(gt_tuning{1, 0, 0, 0, tw, lds});
   for (uint32_t tw : {8u, 16u})

@@ -439,7 +439,7 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
   uint32_t cap = comp_cap ? comp_cap : 64u;
   if (*last_cap > cap) cap = *last_cap;
   gt_tuning tile8{};
-  tile8.path = 1;
+  tile8.path = 3;
   tile8.tile_w = 8;
   tile8.stack_lds = gt_scene_tuning(s)->stack_lds;
   (void)gt_frame_tiles(&tile8, &f, 0);

@@ -88,6 +88,10 @@ struct KBuf {
   uint32_t id[MAXK];
 
   __device__ __forceinline__ void reset(uint32_t off, float t_max) {
+    /* opaque copies of the (uniform) inputs: the 2*MAXK reset values are
+     * recomputed in each round instead of hoisted out of the round loop and
+     * kept live (spilled) across the traversal */
+    asm volatile("" : "+v"(off), "+v"(t_max));
 #pragma unroll
     for (int i = 0; i < MAXK; ++i) {
       const bool dummy = (uint32_t)i < off;
@@ -126,6 +130,19 @@ struct Counters {
 
 enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
 
+/* Read-only scene arrays seen through the constant address space: a load
+ * whose address is the same in every lane becomes one scalar load. */
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4v cfloat4;
+typedef const __attribute__((address_space(4))) u4v cuint4;
+
+template <typename T>
+__device__ __forceinline__ T* as_const(const void* p) {
+  return (T*)(uintptr_t)p;
+}
+
+
 /* Exact proxy test: object-space ray (trace.glsl:102-105), the triangle
  * test of intersect.glsl:73-92 and the facing rule of trace.glsl:153-164.
  * A bounding-sphere test and the Pluecker face pre-test (gt_face_candidates)
@@ -133,14 +150,25 @@ enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
  * normally runs one exact test and one k-buffer insertion per proxy.
  * MODE_CLOSEST keeps back faces (debugMode 1 traces with no cull flag,
  * GS_shader.ts:292-294) and tests all 20 faces. */
-template <int MAXK, int MODE>
+template <typename V4, typename P4>
+__device__ __forceinline__ void load_proxy(P4* P, float* w, uint32_t* gid) {
+  const V4 q0 = P[0], q1 = P[1], q2 = P[2], q3 = P[3];
+  w[0] = q0.x; w[1] = q0.y; w[2] = q0.z; w[3] = q0.w;
+  w[4] = q1.x; w[5] = q1.y; w[6] = q1.z; w[7] = q1.w;
+  w[8] = q2.x; w[9] = q2.y; w[10] = q2.z; w[11] = q2.w;
+  *gid = __float_as_uint(q3.x);
+}
+
+template <int MAXK, int MODE, bool UNIFORM = false>
 __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, const RayState& r, float tmin,
                                            float tmax, KBuf<MAXK>& kb, uint32_t& count, Counters& st,
                                            const float* s_faces) {
-  const float4* P = reinterpret_cast<const float4*>(S.prox) + 4 * (size_t)pi;
-  const float4 q0 = P[0], q1 = P[1], q2 = P[2], q3 = P[3];
-  const float w[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-  const uint32_t gid = __float_as_uint(q3.x);
+  float w[12];
+  uint32_t gid;
+  if (UNIFORM) /* pi is the same in every lane: scalar loads */
+    load_proxy<f4v>(as_const<cfloat4>(S.prox) + 4 * (size_t)pi, w, &gid);
+  else
+    load_proxy<float4>(reinterpret_cast<const float4*>(S.prox) + 4 * (size_t)pi, w, &gid);
   st.proxies++;
   const gt_f3 oo = gt_xform_point(w, r.o);
   const gt_f3 od = gt_xform_dir(w, r.d);
@@ -244,6 +272,85 @@ __device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, floa
   }
 }
 
+/* Wave-wide (packet) traversal for the interval (tmin, tmax) of each lane.
+ * The lanes of a wave walk the BVH together from one LDS stack of
+ * (node ref, lane mask) entries: a popped node is loaded once for the wave
+ * (scalar loads), each lane in the mask tests the four child boxes against
+ * its own ray and k-th entry, and a child is pushed with the mask of the
+ * lanes that hit it.  At a leaf, the lanes in the mask test its proxies.
+ * A lane tests a proxy only if its own box test of the leaf passed, and
+ * may test proxies its own traversal would have culled by the k-th entry;
+ * those hits lie beyond the k-th key and the insertion rejects them, so
+ * every lane ends with exactly the hits of trace().  Children are pushed
+ * far to near by the entry distance of the first lane that hits them. */
+template <int MAXK, int MODE>
+__device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmin, float tmax, float slack_abs,
+                             KBuf<MAXK>& kb, uint32_t& count, Counters& st, uint32_t* wst, const float* s_faces) {
+  if (S.root == GT_NODE_EMPTY) return;
+  const uint32_t lane = __lane_id();
+  const uint64_t live = __ballot(1);
+  const float lo_t = tmin * (1.0f - 2e-5f) - slack_abs;
+  wst[0] = S.root;
+  wst[1] = (uint32_t)live;
+  wst[2] = (uint32_t)(live >> 32);
+  int sp = 1;
+  while (sp > 0) {
+    --sp;
+    const uint32_t ref = __builtin_amdgcn_readfirstlane(wst[3 * sp]);
+    const uint32_t mlo = __builtin_amdgcn_readfirstlane(wst[3 * sp + 1]);
+    const uint32_t mhi = __builtin_amdgcn_readfirstlane(wst[3 * sp + 2]);
+    const bool in = ((lane < 32u ? mlo >> lane : mhi >> (lane - 32u)) & 1u) != 0u;
+    if (ref & GT_LEAF_BIT) {
+      const uint32_t first = ref & GT_LEAF_FIRST_MASK;
+      const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+      if (in)
+        for (uint32_t q = 0; q < cnt; ++q)
+          test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
+      continue;
+    }
+    if (in) st.nodes++;
+    typedef const __attribute__((address_space(4))) float cfloat;
+    cfloat* N = as_const<cfloat>(S.nodes + ref);
+    const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
+    const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
+    float rep[4];
+    uint64_t hm[4];
+    uint32_t cref[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      /* child c: row r of the node holds axis r of the four child boxes */
+      cref[c] = __float_as_uint(N[24 + c]);
+      const float ax = (N[c] - r.o.x) * r.inv.x, bx = (N[4 + c] - r.o.x) * r.inv.x;
+      const float ay = (N[8 + c] - r.o.y) * r.inv.y, by = (N[12 + c] - r.o.y) * r.inv.y;
+      const float az = (N[16 + c] - r.o.z) * r.inv.z, bz = (N[20 + c] - r.o.z) * r.inv.z;
+      const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+      const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+      const bool hit = in && tnear <= tfar && cref[c] != GT_NODE_EMPTY;
+      hm[c] = __ballot(hit);
+      const int fl = hm[c] ? __ffsll((unsigned long long)hm[c]) - 1 : 0;
+      rep[c] = hm[c] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tnear), fl)) : INFINITY;
+    }
+    uint32_t ord[4] = {0u, 1u, 2u, 3u};
+#define GT_OSWAP(a, b)                                                     \
+  if (rep[ord[a]] < rep[ord[b]]) {                                         \
+    const uint32_t o_ = ord[a]; ord[a] = ord[b]; ord[b] = o_;              \
+  }
+    GT_OSWAP(0, 1) GT_OSWAP(2, 3) GT_OSWAP(0, 2) GT_OSWAP(1, 3) GT_OSWAP(1, 2)
+#undef GT_OSWAP
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t k = ord[c];
+      const uint64_t m = hm[k];
+      if (m) {
+        wst[3 * sp] = cref[k];
+        wst[3 * sp + 1] = (uint32_t)m;
+        wst[3 * sp + 2] = (uint32_t)(m >> 32);
+        ++sp;
+      }
+    }
+  }
+}
+
 /* shToRadiance + computeRadiance (GS_shader.ts:138-190): the 16 real-SH
  * basis values with the constants of GS_shader.ts:39-52, then
  * L_c = max(0, 0.5 + sum_j b_j sh[j][c]) with the coefficients streamed
@@ -312,7 +419,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
  * remapped so that consecutive tiles share an XCD.  BATCH: global tiles of
  * all cameras are numbered consecutively (camera c owns tiles from
  * cams[c].tile_base). */
-template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH>
+template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH, bool PACKET>
 __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
                                                                     const gt_dev_frame* __restrict__ cams,
                                                                     uint32_t n_cams, uint32_t total_tiles,
@@ -325,6 +432,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
   Stack stk;
   stk.lds = s_stack + threadIdx.x;
   stk.sp = 0;
+  uint32_t* const wst = s_stack + (threadIdx.x >> 6) * (3u * GT_STACK_MAX); /* PACKET: this wave's stack */
   stk.depth = (int)lds_depth;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tile = tile_offset + blk * 4u + (threadIdx.x >> 6);
@@ -362,7 +470,8 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
       while (F.T_min < T && t_curr < F.t_max) {
         if (F.early_stop == step++) break;
         kb.reset(off, F.t_max);
-        trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
+        if (PACKET) trace_packet<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+        else trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
         t_curr = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
         /* bring the active slots to the front (uniform shifts) */
         if (off & 1u) kb.shift_left(1);
@@ -372,6 +481,9 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
         if (off & 16u) kb.shift_left(16);
         bool stop = false;
         uint32_t mask = 0;
+        /* one copy of the evaluation code: the k-buffer rotates by one slot
+         * per entry (static register moves), so the loop need not unroll */
+#pragma unroll 1
         for (uint32_t j = 0; j < k; ++j) {
           const uint32_t id = kb.id[0];
           if (DUMP && rounds < MR) F.hit_ids[((size_t)ray * MR + rounds) * k + j] = id;
@@ -411,7 +523,8 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
     } else if (MODE == MODE_CLOSEST) {
       /* debugMode 1 (GS_shader.ts:289-296, 312-321) */
       kb.reset(0, F.t_max);
-      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
+      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+      else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t id = kb.id[MAXK - 1];
       if (id != GT_SENTINEL) L = radiance(S.sh + 48 * (size_t)id, F.sh_degree, r.d);
       rounds = 1;
@@ -422,7 +535,8 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
     } else {
       /* debugMode 2 (GS_shader.ts:201-228): front-face hits in (t_min, t_max) */
       kb.reset(0, F.t_max);
-      trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
+      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+      else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t c = count;
       float cr, cg, cb;
       if (c == 0) { cr = 0.f; cg = 0.f; cb = 0.f; }
@@ -641,6 +755,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
         bool stop = false;
         uint32_t mask = 0;
         const size_t MR = DUMP ? CAM(max_rounds) : 0;
+#pragma unroll 1
         for (uint32_t jj = 0; jj < k; ++jj) {
           const uint32_t id = kb.id[0];
           if (DUMP && rounds < MR) CAM(hit_ids)[((size_t)pix * MR + rounds) * k + jj] = id;
@@ -734,19 +849,30 @@ __global__ __launch_bounds__(256) void k_colors(const gt_dev_scene S, float ox, 
   out[i] = make_float4(c.x, c.y, c.z, 0.0f);
 }
 
+template <int MAXK, int MODE, bool BATCH, bool PACKET>
+hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
+                          uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
+                          bool stats, uint32_t tile_offset) {
+  const dim3 grid((total_tiles - tile_offset + 3u) / 4u), block(GT_BLOCK);
+  const size_t lds = PACKET ? (size_t)(GT_BLOCK / 64) * 3u * GT_STACK_MAX * sizeof(uint32_t)
+                            : (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
+#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset
+  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
+  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
+  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
+  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
+#undef GT_ARGS
+  return hipGetLastError();
+}
+
 template <int MAXK, int MODE, bool BATCH>
 hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                         uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                        bool stats, uint32_t tile_offset) {
-  const dim3 grid((total_tiles - tile_offset + 3u) / 4u), block(GT_BLOCK);
-  const size_t lds = (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
-#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset
-  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH>), grid, block, lds, st, GT_ARGS);
-  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH>), grid, block, lds, st, GT_ARGS);
-  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH>), grid, block, lds, st, GT_ARGS);
-  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false, BATCH>), grid, block, lds, st, GT_ARGS);
-#undef GT_ARGS
-  return hipGetLastError();
+                        bool stats, uint32_t tile_offset, bool packet) {
+  return packet ? launch_tiles_p<MAXK, MODE, BATCH, true>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
+                                                          dump, stats, tile_offset)
+                : launch_tiles_p<MAXK, MODE, BATCH, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
+                                                           dump, stats, tile_offset);
 }
 
 template <int MAXK, bool BATCH>
@@ -839,20 +965,22 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   const bool dump = F->hit_ids != nullptr && F->max_rounds > 0;
   const bool stats = F->stats != nullptr;
   const bool batch = n_cams > 1;
+  /* default: packet traversal (path 0 or 3); path 1: per-lane traversal */
+  const bool packet = !tune || tune->path == 0 || tune->path == 3;
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
 #define GT_TILES(MK, MODE)                                                                                       \
   (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                       tile_offset)                                                              \
+                                       tile_offset, packet)                                                      \
          : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                        tile_offset))
+                                        tile_offset, packet))
   if (F->debug_mode == 1) {
     e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                             tile_offset);
+                                             tile_offset, packet);
   } else if (F->debug_mode == 2) {
     e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                           tile_offset);
+                                           tile_offset, packet);
   } else if (F->k <= 4) {
     e = GT_TILES(4, MODE_RENDER);
   } else if (F->k <= 8) {

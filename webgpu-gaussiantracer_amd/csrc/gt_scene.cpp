@@ -683,7 +683,7 @@ extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
   const bool tile_ok = t->tile_w == 0 || t->tile_w == 4 || t->tile_w == 8 || t->tile_w == 16 || t->tile_w == 32;
   const bool lds_ok = t->stack_lds == 0 || t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32 ||
                       t->stack_lds == 64;
-  if (t->path > 2 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
+  if (t->path > 3 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
     gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u, tile_w %u, "
                  "stack_lds %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w, t->stack_lds);
     return GT_E_INVALID;
