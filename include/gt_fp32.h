@@ -256,8 +256,67 @@ GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
 GT_CONST_TABLE uint8_t gt_ico_edge_first[30] = {0,  1,  2,  4,  5,  7,  8,  10, 11, 13, 16, 17, 19, 20, 22,
                                                 23, 25, 26, 28, 29, 30, 31, 32, 34, 35, 37, 38, 40, 41, 43};
 
-GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, const float* edges, const float* normals, float margin_g,
-                                  float margin_n) {
+/* The edge constants {a x b, b - a} of gt_ico_edge_tables and the face
+ * normals of gt_ico_face as literals (checked equal to the computed tables
+ * by the CPU tests and by the library at its first launch): in the
+ * unrolled pre-test each one becomes an instruction literal rather than a
+ * value held in a register for the whole kernel. */
+#define GT_ICO_EDGE_VALUES \
+  {0x1.6a99b6p-1, 0x1.c032e4p-2, 0x1.255994p+0, -0x1.a2b1e4p-2, -0x1.120a14p+0, 0x1.52bb36p-1, \
+   0x1.c032e4p-2, -0x1.255994p+0, 0x1.6a99b6p-1, -0x1.120a14p+0, -0x1.52bb36p-1, -0x1.a2b1e4p-2, \
+   0x1.255994p+0, 0x1.6a99b6p-1, -0x1.c032e4p-2, 0x1.52bb36p-1, -0x1.a2b1e4p-2, 0x1.120a14p+0, \
+   0x1.255994p+0, -0x1.6a99b6p-1, 0x1.c032e4p-2, -0x1.52bb36p-1, -0x1.a2b1e4p-2, 0x1.120a14p+0, \
+   0x0p+0, 0x0p+0, -0x1.6a99b6p+0, 0x1.52bb36p+0, 0x0p+0, 0x0p+0, \
+   -0x1.255994p+0, 0x1.6a99b6p-1, 0x1.c032e4p-2, -0x1.52bb36p-1, -0x1.a2b1e4p-2, -0x1.120a14p+0, \
+   -0x1.255994p+0, -0x1.6a99b6p-1, -0x1.c032e4p-2, 0x1.52bb36p-1, -0x1.a2b1e4p-2, -0x1.120a14p+0, \
+   -0x1.c032e4p-2, 0x1.255994p+0, 0x1.6a99b6p-1, -0x1.120a14p+0, -0x1.52bb36p-1, 0x1.a2b1e4p-2, \
+   -0x1.6a99b6p-1, -0x1.c032e4p-2, 0x1.255994p+0, -0x1.a2b1e4p-2, -0x1.120a14p+0, -0x1.52bb36p-1, \
+   0x0p+0, 0x1.6a99b6p+0, 0x0p+0, 0x0p+0, 0x0p+0, 0x1.52bb36p+0, \
+   0x1.c032e4p-2, 0x1.255994p+0, -0x1.6a99b6p-1, 0x1.120a14p+0, -0x1.52bb36p-1, -0x1.a2b1e4p-2, \
+   0x1.6a99b6p-1, -0x1.c032e4p-2, -0x1.255994p+0, 0x1.a2b1e4p-2, -0x1.120a14p+0, 0x1.52bb36p-1, \
+   -0x1.c032e4p-2, -0x1.255994p+0, -0x1.6a99b6p-1, -0x1.120a14p+0, 0x1.52bb36p-1, -0x1.a2b1e4p-2, \
+   -0x1.6a99b6p+0, 0x0p+0, 0x0p+0, 0x0p+0, 0x1.52bb36p+0, 0x0p+0, \
+   0x1.6a99b6p-1, -0x1.c032e4p-2, -0x1.255994p+0, -0x1.a2b1e4p-2, 0x1.120a14p+0, -0x1.52bb36p-1, \
+   -0x1.6a99b6p-1, 0x1.c032e4p-2, -0x1.255994p+0, -0x1.a2b1e4p-2, 0x1.120a14p+0, 0x1.52bb36p-1, \
+   0x1.6a99b6p+0, 0x0p+0, 0x0p+0, 0x0p+0, 0x1.52bb36p+0, 0x0p+0, \
+   0x1.c032e4p-2, 0x1.255994p+0, -0x1.6a99b6p-1, -0x1.120a14p+0, 0x1.52bb36p-1, 0x1.a2b1e4p-2, \
+   -0x1.6a99b6p-1, 0x1.c032e4p-2, -0x1.255994p+0, 0x1.a2b1e4p-2, -0x1.120a14p+0, -0x1.52bb36p-1, \
+   -0x1.c032e4p-2, -0x1.255994p+0, -0x1.6a99b6p-1, 0x1.120a14p+0, -0x1.52bb36p-1, 0x1.a2b1e4p-2, \
+   -0x1.6a99b6p-1, -0x1.c032e4p-2, 0x1.255994p+0, 0x1.a2b1e4p-2, 0x1.120a14p+0, 0x1.52bb36p-1, \
+   -0x1.c032e4p-2, 0x1.255994p+0, 0x1.6a99b6p-1, 0x1.120a14p+0, 0x1.52bb36p-1, -0x1.a2b1e4p-2, \
+   -0x1.255994p+0, -0x1.6a99b6p-1, -0x1.c032e4p-2, -0x1.52bb36p-1, 0x1.a2b1e4p-2, 0x1.120a14p+0, \
+   -0x1.255994p+0, 0x1.6a99b6p-1, 0x1.c032e4p-2, 0x1.52bb36p-1, 0x1.a2b1e4p-2, 0x1.120a14p+0, \
+   0x0p+0, 0x0p+0, 0x1.6a99b6p+0, 0x1.52bb36p+0, 0x0p+0, 0x0p+0, \
+   0x1.255994p+0, -0x1.6a99b6p-1, 0x1.c032e4p-2, 0x1.52bb36p-1, 0x1.a2b1e4p-2, -0x1.120a14p+0, \
+   0x1.255994p+0, 0x1.6a99b6p-1, -0x1.c032e4p-2, -0x1.52bb36p-1, 0x1.a2b1e4p-2, -0x1.120a14p+0, \
+   0x1.c032e4p-2, -0x1.255994p+0, 0x1.6a99b6p-1, 0x1.120a14p+0, 0x1.52bb36p-1, 0x1.a2b1e4p-2, \
+   0x1.6a99b6p-1, 0x1.c032e4p-2, 0x1.255994p+0, 0x1.a2b1e4p-2, 0x1.120a14p+0, -0x1.52bb36p-1, \
+   0x0p+0, -0x1.6a99b6p+0, 0x0p+0, 0x0p+0, 0x0p+0, 0x1.52bb36p+0}
+#define GT_ICO_FACE_NORMALS \
+  {-0x1.c032e4p-1, 0x1.c032e4p-1, 0x1.c032e4p-1, \
+   0x0p+0, 0x1.6a99b6p+0, 0x1.150088p-1, \
+   0x0p+0, 0x1.6a99b6p+0, -0x1.150088p-1, \
+   -0x1.c032e4p-1, 0x1.c032e4p-1, -0x1.c032e4p-1, \
+   -0x1.6a99b6p+0, 0x1.150088p-1, 0x0p+0, \
+   0x1.c032e4p-1, 0x1.c032e4p-1, 0x1.c032e4p-1, \
+   -0x1.150088p-1, 0x0p+0, 0x1.6a99b6p+0, \
+   -0x1.6a99b6p+0, -0x1.150088p-1, -0x0p+0, \
+   -0x1.150088p-1, 0x0p+0, -0x1.6a99b6p+0, \
+   0x1.c032e4p-1, 0x1.c032e4p-1, -0x1.c032e4p-1, \
+   0x1.c032e4p-1, -0x1.c032e4p-1, 0x1.c032e4p-1, \
+   0x0p+0, -0x1.6a99b6p+0, 0x1.150088p-1, \
+   -0x0p+0, -0x1.6a99b6p+0, -0x1.150088p-1, \
+   0x1.c032e4p-1, -0x1.c032e4p-1, -0x1.c032e4p-1, \
+   0x1.6a99b6p+0, -0x1.150088p-1, 0x0p+0, \
+   0x1.150088p-1, 0x0p+0, 0x1.6a99b6p+0, \
+   -0x1.c032e4p-1, -0x1.c032e4p-1, 0x1.c032e4p-1, \
+   -0x1.c032e4p-1, -0x1.c032e4p-1, -0x1.c032e4p-1, \
+   0x1.150088p-1, 0x0p+0, -0x1.6a99b6p+0, \
+   0x1.6a99b6p+0, 0x1.150088p-1, 0x0p+0}
+
+GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, float margin_g, float margin_n) {
+  const float edges[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
+  const float normals[3 * GT_NUM_FACES] = GT_ICO_FACE_NORMALS;
   gt_f3 w = gt_cross(oo, od);
   float g[GT_NUM_EDGES];
   uint32_t mask = 0;
@@ -280,6 +339,25 @@ GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, const float* edges, const 
     mask |= (uint32_t)ok << f;
   }
   return mask;
+}
+
+/* 1 if the literal tables equal the computed ones (host check). */
+static inline int gt_ico_literals_match(void) {
+  float e[6 * GT_NUM_EDGES];
+  int8_t fe[3 * GT_NUM_FACES];
+  const float le[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
+  const float ln[3 * GT_NUM_FACES] = GT_ICO_FACE_NORMALS;
+  const int8_t lf[3 * GT_NUM_FACES] = GT_ICO_FACE_EDGES;
+  gt_ico_edge_tables(e, fe);
+  for (int i = 0; i < 6 * GT_NUM_EDGES; ++i)
+    if (e[i] != le[i]) return 0;
+  for (int f = 0; f < GT_NUM_FACES; ++f) {
+    gt_face fc = gt_ico_face(f);
+    if (fc.n.x != ln[3 * f] || fc.n.y != ln[3 * f + 1] || fc.n.z != ln[3 * f + 2]) return 0;
+  }
+  for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
+    if (fe[i] != lf[i]) return 0;
+  return 1;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -32,12 +32,10 @@
 
 namespace {
 
-/* Icosahedron tables, uploaded once per device from gt_ico_face /
- * gt_ico_edge_tables: face constants {p0, e0, e1, n}, Pluecker edge
- * constants {a x b, b - a}, signed edge codes per face, face normals. */
+/* Icosahedron face constants {p0, e0, e1, n} (gt_ico_face), uploaded once
+ * per device and copied to LDS by each block.  The pre-test's edge and
+ * normal constants are instruction literals (gt_face_candidates). */
 __constant__ float c_faces[12 * GT_NUM_FACES];
-__constant__ float c_edges[6 * GT_NUM_EDGES];
-__constant__ float c_normals[3 * GT_NUM_FACES];
 
 #define GT_BLOCK 256
 #ifndef GT_MIN_WAVES
@@ -186,7 +184,7 @@ __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, c
   } else {
     float mg, mn;
     gt_face_margins(oo, od, w, &mg, &mn);
-    cand = gt_face_candidates(oo, od, c_edges, c_normals, mg, mn);
+    cand = gt_face_candidates(oo, od, mg, mn);
   }
   while (cand) {
     const int f = __ffs(cand) - 1;
@@ -916,16 +914,8 @@ hipError_t ensure_faces() {
                          fc.e1.x, fc.e1.y, fc.e1.z, fc.n.x,  fc.n.y,  fc.n.z};
     for (int j = 0; j < 12; ++j) f[12 * i + j] = v[j];
   }
-  float edges[6 * GT_NUM_EDGES], normals[3 * GT_NUM_FACES];
-  int8_t face_edges[3 * GT_NUM_FACES];
-  gt_ico_edge_tables(edges, face_edges);
-  for (int i = 0; i < GT_NUM_FACES; ++i)
-    for (int j = 0; j < 3; ++j) normals[3 * i + j] = f[12 * i + 9 + j];
+  if (!gt_ico_literals_match()) return hipErrorInvalidValue; /* literal tables out of date */
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_faces), f, sizeof(f), 0, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_edges), edges, sizeof(edges), 0, hipMemcpyHostToDevice);
-  for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
-    if (face_edges[i] != gt_ico_face_edges[i]) return hipErrorInvalidValue; /* literal table out of date */
-  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_normals), normals, sizeof(normals), 0, hipMemcpyHostToDevice);
   if (e == hipSuccess && dev >= 0 && dev < 64) g_faces_ready[dev] = true;
   return e;
 }

@@ -853,15 +853,7 @@ extern "C" int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, f
 extern "C" int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_t n, float tmin, float tmax,
                                   float margin_scale, uint32_t* cand, uint32_t* exact) {
   if (!w2o || !o || !d || !cand || !exact) { gt_set_error("gt_debug_face_test: NULL argument"); return GT_E_INVALID; }
-  float edges[6 * GT_NUM_EDGES], normals[3 * GT_NUM_FACES];
-  int8_t face_edges[3 * GT_NUM_FACES];
-  gt_ico_edge_tables(edges, face_edges);
-  for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
-    if (face_edges[i] != gt_ico_face_edges[i]) { gt_set_error("face-edge table out of date"); return GT_E_LIMIT; }
-  for (int f = 0; f < GT_NUM_FACES; ++f) {
-    const gt_face fc = gt_ico_face(f);
-    normals[3 * f] = fc.n.x; normals[3 * f + 1] = fc.n.y; normals[3 * f + 2] = fc.n.z;
-  }
+  if (!gt_ico_literals_match()) { gt_set_error("icosahedron literal tables out of date"); return GT_E_LIMIT; }
   for (uint32_t i = 0; i < n; ++i) {
     const float* m = w2o + 12 * (size_t)i;
     const gt_f3 ro = gt_f3_make(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
@@ -869,7 +861,7 @@ extern "C" int gt_debug_face_test(const float* w2o, const float* o, const float*
     const gt_f3 oo = gt_xform_point(m, ro), od = gt_xform_dir(m, rd);
     float mg, mn;
     gt_face_margins(oo, od, m, &mg, &mn);
-    cand[i] = gt_face_candidates(oo, od, edges, normals, mg * margin_scale, mn * margin_scale);
+    cand[i] = gt_face_candidates(oo, od, mg * margin_scale, mn * margin_scale);
     uint32_t ex = 0;
     for (int f = 0; f < GT_NUM_FACES; ++f) {
       const gt_face fc = gt_ico_face(f);
