@@ -301,9 +301,10 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
     if (ref & GT_LEAF_BIT) {
       const uint32_t first = ref & GT_LEAF_FIRST_MASK;
       const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
-      if (in)
-        for (uint32_t q = 0; q < cnt; ++q)
-          test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
+      /* uniform loop, lane predicate inside: a divergent loop around the
+       * k-buffer updates made the compiler copy (and spill) the buffer */
+      for (uint32_t q = 0; q < cnt; ++q)
+        if (in) test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
       continue;
     }
     if (in) st.nodes++;

@@ -20,7 +20,9 @@
 
 #define GT_NODE_EMPTY 0xffffffffu
 #define GT_LEAF_BIT 0x80000000u
-#define GT_LEAF_MAX 8u                /* proxies per leaf */
+#ifndef GT_LEAF_MAX
+#define GT_LEAF_MAX 8u                /* proxies per leaf (at most 8: 3-bit count) */
+#endif
 #define GT_LEAF_COUNT_SHIFT 28        /* leaf ref: 1 | (count-1):3 | first:28 */
 #define GT_LEAF_FIRST_MASK 0x0fffffffu
 #define GT_MAX_PROXIES GT_LEAF_FIRST_MASK

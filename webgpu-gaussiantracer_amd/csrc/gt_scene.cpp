@@ -210,6 +210,11 @@ static Box proxy_box(const float* m) {
   return b;
 }
 
+/* SAH cost of one proxy test relative to one box test */
+#ifndef GT_SAH_ISECT
+#define GT_SAH_ISECT 2.0
+#endif
+
 struct Builder {
   const std::vector<Box>& boxes;
   std::vector<float> cent;  // 3 per prim
@@ -235,7 +240,7 @@ struct Builder {
     nodes.emplace_back();
     work.push_back({0, 0, n, 0});
     constexpr int NB = 32;
-    const double C_TRAV = 1.0, C_ISECT = 2.0;
+    const double C_TRAV = 1.0, C_ISECT = GT_SAH_ISECT;
     while (!work.empty()) {
       Task t = work.back();
       work.pop_back();
