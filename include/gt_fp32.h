@@ -247,14 +247,13 @@ GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GT_UNROLL _Pragma("unroll")
+/* keeps the scheduler from evaluating later faces' edge values early (which
+ * would keep all 30 live at once) */
+#define GT_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
 #else
 #define GT_UNROLL
+#define GT_SCHED_BARRIER()
 #endif
-
-/* Slot 3f+j at which edge e is first used by face f; edge values are
- * computed there, so at most 16 of the 30 are live at once. */
-GT_CONST_TABLE uint8_t gt_ico_edge_first[30] = {0,  1,  2,  4,  5,  7,  8,  10, 11, 13, 16, 17, 19, 20, 22,
-                                                23, 25, 26, 28, 29, 30, 31, 32, 34, 35, 37, 38, 40, 41, 43};
 
 /* The edge constants {a x b, b - a} of gt_ico_edge_tables and the face
  * normals of gt_ico_face as literals (checked equal to the computed tables
@@ -314,31 +313,37 @@ GT_CONST_TABLE uint8_t gt_ico_edge_first[30] = {0,  1,  2,  4,  5,  7,  8,  10, 
    0x1.150088p-1, 0x0p+0, -0x1.6a99b6p+0, \
    0x1.6a99b6p+0, 0x1.150088p-1, 0x0p+0}
 
+/* Faces of each edge: edge e is used with + orientation by face
+ * GT_ICO_EDGE_FACES[2e] and with - orientation by face [2e+1] (the inverse
+ * of GT_ICO_FACE_EDGES; every edge borders two faces). */
+#define GT_ICO_EDGE_FACES \
+  {0, 4, 6, 0, 1, 0, 5, 1, 2, 1, 2, 9, 3, 2, 3, 8, 4, 3, 4, 7, 5, 15, 19, 5, 16, 6, 6, 15, 17, 7, \
+   7, 16, 18, 8, 8, 17, 9, 19, 18, 9, 10, 14, 15, 10, 11, 10, 16, 11, 11, 12, 12, 17, 13, 12, 13, 18, 14, 13, 14, 19}
+
+/* Edge-major evaluation: each edge value is formed once and immediately
+ * marks the faces whose test it fails (!(v <= margin_g), so a NaN fails as
+ * in the face-major form), so only one edge value is live at a time. */
 GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, float margin_g, float margin_n) {
   const float edges[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
   const float normals[3 * GT_NUM_FACES] = GT_ICO_FACE_NORMALS;
+  const uint8_t ef[2 * GT_NUM_EDGES] = GT_ICO_EDGE_FACES;
   gt_f3 w = gt_cross(oo, od);
-  float g[GT_NUM_EDGES];
-  uint32_t mask = 0;
+  uint32_t bad = 0;
   GT_UNROLL
   for (int f = 0; f < GT_NUM_FACES; ++f) {
     const float* n = normals + 3 * f;
     float nd = (n[0] * od.x + n[1] * od.y) + n[2] * od.z;
-    int ok = nd <= margin_n;
-    GT_UNROLL
-    for (int j = 0; j < 3; ++j) {
-      int c = gt_ico_face_edges[3 * f + j];
-      int e = (c > 0 ? c : -c) - 1;
-      if (gt_ico_edge_first[e] == 3 * f + j) {
-        const float* E = edges + 6 * e;
-        g[e] = ((od.x * E[0] + od.y * E[1]) + od.z * E[2]) + ((w.x * E[3] + w.y * E[4]) + w.z * E[5]);
-      }
-      float v = c > 0 ? g[e] : -g[e];
-      ok = ok && (v <= margin_g);
-    }
-    mask |= (uint32_t)ok << f;
+    bad |= (uint32_t)!(nd <= margin_n) << f;
   }
-  return mask;
+  GT_UNROLL
+  for (int e = 0; e < GT_NUM_EDGES; ++e) {
+    const float* E = edges + 6 * e;
+    float g = ((od.x * E[0] + od.y * E[1]) + od.z * E[2]) + ((w.x * E[3] + w.y * E[4]) + w.z * E[5]);
+    bad |= (uint32_t)!(g <= margin_g) << ef[2 * e];
+    bad |= (uint32_t)!(-g <= margin_g) << ef[2 * e + 1];
+    GT_SCHED_BARRIER();
+  }
+  return ~bad & ((1u << GT_NUM_FACES) - 1u);
 }
 
 /* 1 if the literal tables equal the computed ones (host check). */
@@ -357,6 +362,14 @@ static inline int gt_ico_literals_match(void) {
   }
   for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
     if (fe[i] != lf[i]) return 0;
+  {
+    const uint8_t ef[2 * GT_NUM_EDGES] = GT_ICO_EDGE_FACES;
+    for (int f = 0; f < GT_NUM_FACES; ++f)
+      for (int j = 0; j < 3; ++j) {
+        const int c = lf[3 * f + j], e = (c > 0 ? c : -c) - 1;
+        if (ef[2 * e + (c > 0 ? 0 : 1)] != f) return 0;
+      }
+  }
   return 1;
 }
 

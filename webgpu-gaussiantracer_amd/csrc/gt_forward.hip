@@ -292,6 +292,7 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
   wst[1] = (uint32_t)live;
   wst[2] = (uint32_t)(live >> 32);
   int sp = 1;
+  float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1]; /* this lane's k-th distance */
   while (sp > 0) {
     --sp;
     const uint32_t ref = __builtin_amdgcn_readfirstlane(wst[3 * sp]);
@@ -305,46 +306,46 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
        * k-buffer updates made the compiler copy (and spill) the buffer */
       for (uint32_t q = 0; q < cnt; ++q)
         if (in) test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
-      continue;
-    }
-    if (in) st.nodes++;
-    typedef const __attribute__((address_space(4))) float cfloat;
-    cfloat* N = as_const<cfloat>(S.nodes + ref);
-    const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
-    const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
-    float rep[4];
-    uint64_t hm[4];
-    uint32_t cref[4];
+      if (MODE != MODE_COUNT) kth = kb.t[MAXK - 1];
+    } else {
+      if (in) st.nodes++;
+      typedef const __attribute__((address_space(4))) float cfloat;
+      cfloat* N = as_const<cfloat>(S.nodes + ref);
+      const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
+      float rep[4];
+      uint64_t hm[4];
+      uint32_t cref[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      /* child c: row r of the node holds axis r of the four child boxes */
-      cref[c] = __float_as_uint(N[24 + c]);
-      const float ax = (N[c] - r.o.x) * r.inv.x, bx = (N[4 + c] - r.o.x) * r.inv.x;
-      const float ay = (N[8 + c] - r.o.y) * r.inv.y, by = (N[12 + c] - r.o.y) * r.inv.y;
-      const float az = (N[16 + c] - r.o.z) * r.inv.z, bz = (N[20 + c] - r.o.z) * r.inv.z;
-      const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
-      const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
-      const bool hit = in && tnear <= tfar && cref[c] != GT_NODE_EMPTY;
-      hm[c] = __ballot(hit);
-      const int fl = hm[c] ? __ffsll((unsigned long long)hm[c]) - 1 : 0;
-      rep[c] = hm[c] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tnear), fl)) : INFINITY;
-    }
-    uint32_t ord[4] = {0u, 1u, 2u, 3u};
+      for (int c = 0; c < 4; ++c) {
+        /* child c: row r of the node holds axis r of the four child boxes */
+        cref[c] = __float_as_uint(N[24 + c]);
+        const float ax = (N[c] - r.o.x) * r.inv.x, bx = (N[4 + c] - r.o.x) * r.inv.x;
+        const float ay = (N[8 + c] - r.o.y) * r.inv.y, by = (N[12 + c] - r.o.y) * r.inv.y;
+        const float az = (N[16 + c] - r.o.z) * r.inv.z, bz = (N[20 + c] - r.o.z) * r.inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
+        const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
+        const bool hit = in && tnear <= tfar && cref[c] != GT_NODE_EMPTY;
+        hm[c] = __ballot(hit);
+        const int fl = hm[c] ? __ffsll((unsigned long long)hm[c]) - 1 : 0;
+        rep[c] = hm[c] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tnear), fl)) : INFINITY;
+      }
+      uint32_t ord[4] = {0u, 1u, 2u, 3u};
 #define GT_OSWAP(a, b)                                                     \
-  if (rep[ord[a]] < rep[ord[b]]) {                                         \
-    const uint32_t o_ = ord[a]; ord[a] = ord[b]; ord[b] = o_;              \
-  }
-    GT_OSWAP(0, 1) GT_OSWAP(2, 3) GT_OSWAP(0, 2) GT_OSWAP(1, 3) GT_OSWAP(1, 2)
+    if (rep[ord[a]] < rep[ord[b]]) {                                         \
+      const uint32_t o_ = ord[a]; ord[a] = ord[b]; ord[b] = o_;              \
+    }
+      GT_OSWAP(0, 1) GT_OSWAP(2, 3) GT_OSWAP(0, 2) GT_OSWAP(1, 3) GT_OSWAP(1, 2)
 #undef GT_OSWAP
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const uint32_t k = ord[c];
-      const uint64_t m = hm[k];
-      if (m) {
-        wst[3 * sp] = cref[k];
-        wst[3 * sp + 1] = (uint32_t)m;
-        wst[3 * sp + 2] = (uint32_t)(m >> 32);
-        ++sp;
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t k = ord[c];
+        const uint64_t m = hm[k];
+        if (m) {
+          wst[3 * sp] = cref[k];
+          wst[3 * sp + 1] = (uint32_t)m;
+          wst[3 * sp + 2] = (uint32_t)(m >> 32);
+          ++sp;
+        }
       }
     }
   }
