@@ -190,10 +190,10 @@ GT_HD gt_face gt_ico_face(int f) {
  * 1 - beta - gamma = F(p1->p2) / dot(n, d) (the three F sum to dot(n, d)),
  * so a FRONT face (dot(n, d) < 0) can pass only if all three F <= 0.
  * gt_face_candidates evaluates the 30 edge values once, without division,
- * and returns the mask of faces whose three values are <= margin_g and whose
- * dot(n, d) <= margin_n.  With margin_g far above the FP32 error of the
- * triangle test, every face the exact test + facing rule accepts is in the
- * mask; the exact test then runs only on those faces. */
+ * and returns the mask of faces whose three signed values are <= margin_g.
+ * With margin_g far above the FP32 error of the triangle test, every face
+ * the exact test + facing rule accepts is in the mask; the exact test then
+ * runs only on those faces. */
 #define GT_NUM_EDGES 30
 
 /* edges: 30 x 6 floats {a x b, b - a} per undirected edge (a < b);
@@ -321,20 +321,18 @@ GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
    7, 16, 18, 8, 8, 17, 9, 19, 18, 9, 10, 14, 15, 10, 11, 10, 16, 11, 11, 12, 12, 17, 13, 12, 13, 18, 14, 13, 14, 19}
 
 /* Edge-major evaluation: each edge value is formed once and immediately
- * marks the faces whose test it fails (!(v <= margin_g), so a NaN fails as
- * in the face-major form), so only one edge value is live at a time. */
+ * marks the faces whose test it fails (!(v <= margin_g): a NaN fails), so
+ * only one edge value is live at a time.  No facing pre-test: a back face
+ * passes "all three signed values <= margin_g" only when all three lie in
+ * [0, margin_g] (its three F sum to dot(n, d) > 0 with the opposite sign
+ * pattern), i.e. for faces nearly parallel to the ray, and the exact test's
+ * facing rule rejects it; margin_n is kept for the interface. */
 GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, float margin_g, float margin_n) {
   const float edges[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
-  const float normals[3 * GT_NUM_FACES] = GT_ICO_FACE_NORMALS;
   const uint8_t ef[2 * GT_NUM_EDGES] = GT_ICO_EDGE_FACES;
   gt_f3 w = gt_cross(oo, od);
   uint32_t bad = 0;
-  GT_UNROLL
-  for (int f = 0; f < GT_NUM_FACES; ++f) {
-    const float* n = normals + 3 * f;
-    float nd = (n[0] * od.x + n[1] * od.y) + n[2] * od.z;
-    bad |= (uint32_t)!(nd <= margin_n) << f;
-  }
+  (void)margin_n;
   GT_UNROLL
   for (int e = 0; e < GT_NUM_EDGES; ++e) {
     const float* E = edges + 6 * e;
