@@ -20,6 +20,7 @@ fwd_bwd: forward with the backward checkpoint + dL/dRGBA of a squared-error
 loss + the hand-written backward, on the same frame (and the recompute mode).
 batched: 64 cameras of uneven resolution in one launch replayed from a
 hipGraph, against the same cameras rendered one launch at a time.
+fwd_bwd_c4: the same forward + backward on 200k single-cluster Gaussians, 800x800.
 c5: 3M Gaussians, 16 views at 1280x720, forward + recompute-mode backward
 per view with gradients accumulated across views, peak device memory.
 
@@ -41,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # this script scaled by the counters' reading for a 1 GiB device copy
 # (x2.000 and x1.000): profiles/k_forward_1M_1080p_k16.md.  Reported as
 # roofline.traffic for that workload only.
-TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 1500693900, "write_bytes": 474520272}
+TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 1632464770, "write_bytes": 1801607136}
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
 
 
@@ -238,6 +239,29 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
     }
 
 
+def bench_c4(gt, torch, dev, sp, steps):
+    """Forward + backward on 200,000 single-cluster Gaussians (N(0, 0.8)^3,
+    log-scales N(-3.5, 0.5)), one 800x800 camera at (0, 0.5, 4) looking at
+    the origin, hFov 50, alpha_min 0.01, k 16, T_min 0.001; same step and
+    loss as the 1M-Gaussian forward + backward."""
+    raw = gt.generate_raw(seed=0, config="c4")
+    scene = gt.Scene(raw, alpha_min=0.01)
+    W = H = 800
+    u = gt.make_uniforms(gt.camera_look_at([0.0, 0.5, 4.0], [0.0, 0.0, 0.0]), W, H, hfov_deg=50.0, T_min=1e-3,
+                         alpha_min=0.01, k=16)
+    stream = torch.cuda.current_stream()
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    stats = torch.zeros(8, dtype=torch.int64, device=dev)
+    scene.render_device(u, W, H, rgba.data_ptr(), sp, stats.data_ptr())
+    torch.cuda.synchronize()
+    st = dict(zip(gt.STAT_NAMES, [int(x) for x in stats.cpu().tolist()]))
+    ms_f = _events_ms(torch, stream, lambda: scene.render_device(u, W, H, rgba.data_ptr(), sp), steps)
+    r = bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, 1, ms_f, st)
+    r["ms_forward"] = round(ms_f, 3)
+    r["workload"] = ("200000 Gaussians single cluster, 800x800, k 16: " + r["workload"].split("; ")[0])
+    return r
+
+
 def bench_c5(gt, torch, dev, sp, steps):
     """Batched forward + backward at scale: 3M Gaussians (256-cluster mixture in
     a 40^3 box, log-scales N(-4.5, 0.7)), 16 cameras at 1280x720 on a sphere of
@@ -316,6 +340,7 @@ def main():
     ap.add_argument("--batch-cams", type=int, default=64)
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the 3M-Gaussian 16-view forward+backward")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 200k-Gaussian 800x800 forward+backward")
     ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -410,7 +435,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": (TRAFFIC_DEFAULT_FRAME["fetch_bytes"] + TRAFFIC_DEFAULT_FRAME["write_bytes"]
                         if (args.n, W, H, K) == (1000000, 1920, 1080, 16) else None),
-            "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16,0,false,false>",
+            "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16, 0, false, false, false, true> (packet traversal)",
         },
         "stats": st,
         "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
@@ -436,6 +461,8 @@ def main():
                                        min(args.warmup, 2), ms_step, st)
     if not args.no_batched:
         out["batched"] = bench_batched(gt, scene, torch, dev, sp, K, args.batch_cams)
+    if not args.no_backward and not args.no_c4:
+        out["fwd_bwd_c4"] = bench_c4(gt, torch, dev, sp, 5)
     if not args.no_c5:
         out["c5"] = bench_c5(gt, torch, dev, sp, 2)
 
