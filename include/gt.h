@@ -93,6 +93,24 @@ int gt_scene_create_packed(const void* records, uint32_t n, double alpha_min, gt
 /* Re-filter and rebuild for a new alpha_min (the reference rebuilds the TLAS). */
 int gt_scene_set_alpha_min(gt_scene* s, double alpha_min);
 
+/* The same scene built on the GPU from DEVICE-resident PLY-raw parameters
+ * (raw_dev: n x 59 floats), for callers whose parameters live in HBM (a
+ * training loop).  Replaces the host stages gaussianSceneToBvh
+ * (src/main.ts:30-111) and the wasm TLAS build
+ * (webrtx/src/wasm_bvh_builder.ts:260-377, webrtx/bvh/src/lib.rs:253-347):
+ * activations, records, proxy transforms and boxes are computed by the same
+ * code as gt_scene_create_raw (bit-identical); the BVH is a GPU radix tree
+ * over Morton codes with SAH-chosen leaves, collapsed to the same 4-wide
+ * node format.  Hit lists and pixels equal those of the host-built scene.
+ * raw_dev is copied into the scene; the call returns when the build is done
+ * (it synchronises hip_stream).  gt_scene_set_alpha_min rebuilds on the GPU. */
+int gt_scene_create_device(const float* raw_dev, uint32_t n, double alpha_min, void* hip_stream, gt_scene** out);
+
+/* Rebuilds a gt_scene_create_device scene from updated parameters (same n,
+ * same alpha_min), reusing its device memory: the step after an optimizer
+ * update.  Must not overlap renders or backward calls on the scene. */
+int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* hip_stream);
+
 void gt_scene_destroy(gt_scene* s);
 
 typedef struct gt_scene_info {

@@ -138,6 +138,52 @@ GT_HD double gt_exp64(double x) {
   return p * gt_bits_to_f64((uint64_t)(n + 1023) << 52);
 }
 
+GT_HD uint64_t gt_f64_to_bits(double f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint64_t)__double_as_longlong(f);
+#else
+  uint64_t u; memcpy(&u, &f, 8); return u;
+#endif
+}
+
+/*
+ * gt_log64: natural log in FP64 for finite x > 0 (the activations' ln, used
+ * identically by the host and the device scene builders).  With
+ * x = m 2^e, m in [sqrt(1/2), sqrt(2)):
+ *   s = (m - 1) / (m + 1),  ln m = 2 atanh(s) = 2s + 2s z P(z),  z = s^2,
+ *   P(z) = sum_{i=0..10} z^i / (2i + 3)   (|s| <= 0.1716, truncation < 1e-18 rel.)
+ *   ln x = e LN2_HI + (e LN2_LO + ln m)   (e LN2_HI exact, as in gt_exp64).
+ * m - 1 and m + 1 are exact (Sterbenz); relative error about 1e-16.
+ * x <= 0 or NaN returns NaN, +inf returns +inf; subnormals are rescaled.
+ */
+GT_HD double gt_log64(double x) {
+  if (!(x > 0.0)) return gt_bits_to_f64(0x7ff8000000000000ull);
+  if (x == gt_bits_to_f64(0x7ff0000000000000ull)) return x;
+  int e = 0;
+  if (x < 2.2250738585072014e-308) { x *= 18014398509481984.0; e = -54; } /* 2^54 */
+  uint64_t u = gt_f64_to_bits(x);
+  e += (int)((u >> 52) & 0x7ff) - 1023;
+  double m = gt_bits_to_f64((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull); /* [1, 2) */
+  if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 23.0;
+  p = p * z + 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  const double s2 = s + s;
+  const double lm = s2 + s2 * (z * p);
+  const double fe = (double)e;
+  return (fe * GT_EXP64_LN2_LO + lm) + fe * GT_EXP64_LN2_HI;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Icosahedron proxy (src/main.ts:12-25): 12 vertices, 20 faces, inradius 1,
  * circumradius 1.2584.  Vertex coordinates are the f32 roundings of

@@ -31,6 +31,8 @@ static int g_mutation = 0;
 void gto_set_mutation(int m) { g_mutation = m; }
 
 /* ---------------- activations: src/GS_ply.ts:187-196, 225-235 ---------------- */
+/* exp and log are gt_exp64 / gt_log64 (the library's definitions of the JS
+ * engine's Math.exp / Math.log, which are not correctly rounded either). */
 void gto_activate(const float* raw, uint32_t n, float* packed, double* act8) {
   for (uint32_t i = 0; i < n; ++i) {
     const float* r = raw + (size_t)i * GTO_RAW_FLOATS;
@@ -39,11 +41,11 @@ void gto_activate(const float* raw, uint32_t n, float* packed, double* act8) {
     memset(o, 0, GTO_PACKED_FLOATS * sizeof(float));
     o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
     double sc[3], q[4];
-    for (int j = 0; j < 3; ++j) sc[j] = exp((double)r[52 + j]);
+    for (int j = 0; j < 3; ++j) sc[j] = gt_exp64((double)r[52 + j]);
     double q0 = r[55], q1 = r[56], q2 = r[57], q3 = r[58];
     double norm = sqrt(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
     q[0] = q0 / norm; q[1] = q1 / norm; q[2] = q2 / norm; q[3] = q3 / norm;
-    double op = 1.0 / (1.0 + exp(-(double)r[51]));
+    double op = 1.0 / (1.0 + gt_exp64(-(double)r[51]));
     for (int j = 0; j < 3; ++j) o[4 + j] = (float)sc[j];
     for (int j = 0; j < 4; ++j) o[8 + j] = (float)q[j];
     o[12] = (float)op;
@@ -97,7 +99,7 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
     const float* r = raw + (size_t)i * GTO_RAW_FLOATS;
     double op = a[0];
     if (!(op > alpha_min)) continue;
-    double s = sqrt(2 * log(op / alpha_min));
+    double s = sqrt(2 * gt_log64(op / alpha_min));
     if (g_mutation == 7) s = 1.0;
     float S[3];
     for (int j = 0; j < 3; ++j) S[j] = (float)(s * a[1 + j]);
@@ -115,6 +117,7 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
 
 /* ---------------- shading pieces ---------------- */
 double gto_exp64(double x) { return gt_exp64(x); }
+double gto_log64(double x) { return gt_log64(x); }
 
 void gto_ray_dir(const gto_params* p, uint32_t px, uint32_t py, float* d3) {
   gt_f3 d;

@@ -88,6 +88,7 @@ void gto_set_mutation(int m);
 /* Pieces exposed for pinning tests. */
 void gto_ray_dir(const gto_params* p, uint32_t px, uint32_t py, float* d3);
 double gto_exp64(double x);
+double gto_log64(double x);
 double gto_response(const float* packed_rec, const float* o3, const float* d3);
 void gto_radiance(const float* packed_rec, uint32_t sh_degree, const float* dir3, float* rgb);
 int gto_tri_hit(const float* o3, const float* d3, float tmin, float tmax, int face, float* t);

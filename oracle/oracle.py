@@ -54,6 +54,8 @@ _lib.gto_ray_dir.argtypes = [C.POINTER(Params), C.c_uint32, C.c_uint32, _P]
 _lib.gto_ray_dir.restype = None
 _lib.gto_exp64.argtypes = [C.c_double]
 _lib.gto_exp64.restype = C.c_double
+_lib.gto_log64.argtypes = [C.c_double]
+_lib.gto_log64.restype = C.c_double
 _lib.gto_response.argtypes = [_P, _P, _P]
 _lib.gto_response.restype = C.c_double
 _lib.gto_radiance.argtypes = [_P, C.c_uint32, _P, _P]
@@ -174,6 +176,10 @@ def ray_dir(params, px, py):
 
 def exp64(x):
     return _lib.gto_exp64(float(x))
+
+
+def log64(x):
+    return _lib.gto_log64(float(x))
 
 
 def response(rec, o, d):

@@ -85,6 +85,24 @@ def check_exp64():
     assert orc.exp64(1.0) == math.e or abs(orc.exp64(1.0) / math.e - 1) < 3e-16
 
 
+def check_log64():
+    """gt_log64 (the activations' ln: proxy scale sqrt(2 ln(opacity / alpha_min)),
+    src/main.ts:84) against numpy's log over the double range."""
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([np.logspace(-300, 300, 20001), 1.0 + rng.uniform(-1e-3, 1e-3, 20000),
+                         rng.uniform(1.0, 200.0, 20000), 1.0 + np.logspace(-15, -1, 2000),
+                         [5e-324, 2.2250738585072014e-308, 0.5, 2.0, 1.4142135623730951, 0.7071067811865476]])
+    got = np.array([orc.log64(float(x)) for x in xs])
+    ref = np.log(xs)
+    nz = ref != 0
+    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    assert rel.max() < 6e-16, f"log64 relative error {rel.max()} at x={xs[nz][rel.argmax()]}"
+    assert orc.log64(1.0) == 0.0
+    assert abs(orc.log64(2.0) - math.log(2.0)) <= 1.2e-16
+    assert math.isnan(orc.log64(0.0)) and math.isnan(orc.log64(-1.0)) and math.isnan(orc.log64(float("nan")))
+    assert math.isinf(orc.log64(float("inf")))
+
+
 def check_icosahedron():
     V, I = ico_geometry()
     for f in range(20):
@@ -387,7 +405,7 @@ def check_invariants_and_reeval():
 
 
 PINS = {
-    "exp64": check_exp64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
+    "exp64": check_exp64, "log64": check_log64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
     "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
     "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
     "composite": check_composite_closed_form, "relaunch": check_relaunch_window,

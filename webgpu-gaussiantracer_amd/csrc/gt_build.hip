@@ -1,0 +1,625 @@
+/*
+ * gt_build.hip - scene construction on the GPU (gt_scene_create_device,
+ * gt_scene_update_device).
+ *
+ * Replaces the host stages of the reference's acceleration-container build
+ * for the Gaussian path:
+ *   activations + record packing    src/GS_ply.ts:187-196, 225-235
+ *   gaussianSceneToBvh              src/main.ts:30-111 (filter, proxy transforms)
+ *   wasm TLAS build                 webrtx/src/wasm_bvh_builder.ts:260-377,
+ *                                   webrtx/bvh/src/lib.rs:253-347, 482-505
+ * which the reference runs on the CPU (and which runs out of memory on
+ * large scenes, README.md:50-51).  Everything stays in HBM:
+ *
+ *   k_prep     one thread per Gaussian: activations in double, the 48-B
+ *              geometry and 192-B SH records, proxy transform + inverse and
+ *              padded world box (gt_prep.h, the same code as the host
+ *              builder, so the records are bit-identical), proxy count and
+ *              centroid bounds (wave reductions + atomics)
+ *   k_keys     64-bit key = 30-bit Morton code of the box centroid << 32 |
+ *              Gaussian id (unique keys; non-proxies sort last)
+ *   radix sort rocPRIM radix_sort_keys
+ *   k_gather   64-B proxy records in key order (BVH leaf order)
+ *   k_karras   binary radix tree over the sorted keys (T. Karras, "Maximizing
+ *              parallelism in the construction of BVHs, octrees, and k-d
+ *              trees", HPG 2012), one thread per inner node
+ *   k_refit    bottom-up boxes with one arrival counter per node (the second
+ *              child to arrive continues), and the SAH-optimal leaf decision
+ *              cost = min(C_I n A, C_T A + cost_L + cost_R) for n <= 8
+ *   k_collapse top-down, one launch per level: each 4-wide node opens the
+ *              inner child of largest surface area until it has four
+ *              children (the rule of the host builder); subtrees marked as
+ *              leaves become leaf refs over their contiguous proxy range
+ *   k_need     bottom-up worst-case traversal stack per node
+ *
+ * The node and proxy formats are those of the host builder (gt_internal.h),
+ * so the render and backward kernels run unchanged.  Which hits a ray finds
+ * does not depend on the tree (conservative boxes, (t, id) key order), so a
+ * device-built scene renders exactly what the host-built one renders.
+ */
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "gt_prep.h"
+#include "gt_scene_impl.h"
+
+#define BUILD_BLOCK 256
+#define BIN_LEAF 0x80000000u /* binary ref: leaf = sorted proxy index | BIN_LEAF */
+#define BIN_NONE 0xffffffffu
+
+/* counters (u32) */
+enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OUT = 8, CTL_OVF = 9, CTL_WORDS = 16 };
+
+namespace {
+
+struct BNode {              /* inner node of the binary radix tree (64 B) */
+  float lo[3], hi[3];
+  uint32_t left, right;     /* binary refs */
+  uint32_t parent;          /* BIN_NONE for the root */
+  uint32_t first, count;    /* sorted proxy range */
+  uint32_t leaf;            /* 1: this subtree becomes one leaf (count <= GT_LEAF_MAX) */
+  double cost;              /* SAH cost (area-weighted) */
+  uint32_t _pad[2];
+};
+
+struct Front {              /* collapse work item: binary ref -> 4-wide node index */
+  uint32_t bref, node4;
+};
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float ord2f(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ __forceinline__ uint32_t dev_leaf_ref(uint32_t first, uint32_t count) {
+  return GT_LEAF_BIT | ((count - 1u) << GT_LEAF_COUNT_SHIFT) | first;
+}
+
+/* surface area as the host builder computes it (Box::area) */
+__device__ __forceinline__ double box_area(const float* lo, const float* hi) {
+  const double dx = fmaxf(0.0f, hi[0] - lo[0]), dy = fmaxf(0.0f, hi[1] - lo[1]), dz = fmaxf(0.0f, hi[2] - lo[2]);
+  return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+__global__ void k_init(unsigned int* ctl) {
+  const uint32_t i = threadIdx.x;
+  if (i < CTL_WORDS) {
+    uint32_t v = 0;
+    if (i >= CTL_CMIN && i < CTL_CMIN + 3) v = 0xffffffffu;
+    if (i == CTL_NODES) v = 1u; /* node 0 = root */
+    ctl[i] = v;
+  }
+}
+
+__global__ void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+/* box: 2 float4 per Gaussian {lo.xyz, has_proxy} {hi.xyz, 0} */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ raw, uint32_t n, double alpha_min,
+                                                      float* __restrict__ geom, float* __restrict__ sh,
+                                                      float* __restrict__ o2w, float* __restrict__ w2o,
+                                                      float4* __restrict__ box, unsigned int* __restrict__ ctl) {
+  const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  bool has = false;
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  if (i < n) {
+    float r[GT_RAW_FLOATS];
+    const float* src = raw + (size_t)i * GT_RAW_FLOATS;
+#pragma unroll
+    for (int j = 0; j < GT_RAW_FLOATS; ++j) r[j] = src[j];
+    double act[8];
+    float g[12], s[48];
+    gt_prep_activate(r, act, g, s);
+    float4* G = reinterpret_cast<float4*>(geom) + 3 * (size_t)i;
+    G[0] = make_float4(g[0], g[1], g[2], g[3]);
+    G[1] = make_float4(g[4], g[5], g[6], g[7]);
+    G[2] = make_float4(g[8], g[9], g[10], g[11]);
+    float4* SH = reinterpret_cast<float4*>(sh) + 12 * (size_t)i;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) SH[q] = make_float4(s[4 * q], s[4 * q + 1], s[4 * q + 2], s[4 * q + 3]);
+    float m[12], inv[12];
+    has = gt_prep_proxy(act, r, alpha_min, m, inv) != 0;
+    float4* B = box + 2 * (size_t)i;
+    if (has) {
+      float4* M = reinterpret_cast<float4*>(o2w) + 3 * (size_t)i;
+      float4* V = reinterpret_cast<float4*>(w2o) + 3 * (size_t)i;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        M[q] = make_float4(m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]);
+        V[q] = make_float4(inv[4 * q], inv[4 * q + 1], inv[4 * q + 2], inv[4 * q + 3]);
+      }
+      float lo[3], hi[3];
+      gt_prep_box(m, lo, hi);
+      B[0] = make_float4(lo[0], lo[1], lo[2], 1.0f);
+      B[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) c[j] = 0.5f * (lo[j] + hi[j]);
+    } else {
+      B[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      B[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  }
+  const uint64_t m = __ballot(has);
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t mn[3], mx[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    mn[j] = wave_min_u32(has ? f2ord(c[j]) : 0xffffffffu);
+    mx[j] = wave_max_u32(has ? f2ord(c[j]) : 0u);
+  }
+  if (lane == 0 && m) {
+    atomicAdd(ctl + CTL_NPROX, (unsigned int)__popcll(m));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      atomicMin(ctl + CTL_CMIN + j, mn[j]);
+      atomicMax(ctl + CTL_CMAX + j, mx[j]);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(BUILD_BLOCK) void k_keys(const float4* __restrict__ box, uint32_t n,
+                                                      const unsigned int* __restrict__ ctl,
+                                                      unsigned long long* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float4 b0 = box[2 * (size_t)i], b1 = box[2 * (size_t)i + 1];
+  if (b0.w == 0.0f) {
+    keys[i] = ~0ull;
+    return;
+  }
+  const float lo[3] = {b0.x, b0.y, b0.z}, hi[3] = {b1.x, b1.y, b1.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float cmin = ord2f(ctl[CTL_CMIN + j]), cmax = ord2f(ctl[CTL_CMAX + j]);
+    const float cj = 0.5f * (lo[j] + hi[j]);
+    const float ext = cmax - cmin;
+    const float u = ext > 0.0f ? (cj - cmin) / ext : 0.0f;
+    int v = (int)(u * 1024.0f);
+    v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+    q[j] = (uint32_t)v;
+  }
+  const uint32_t morton = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+  keys[i] = ((unsigned long long)morton << 32) | i;
+}
+
+/* proxies in key order; front[0] = the root work item */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_gather(const unsigned long long* __restrict__ keys, uint32_t np,
+                                                        const float4* __restrict__ w2o, const float4* __restrict__ box,
+                                                        float4* __restrict__ prox, float4* __restrict__ lbox,
+                                                        Front* __restrict__ front) {
+  const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (p == 0) front[0] = Front{np == 1 ? BIN_LEAF : 0u, 0u};
+  if (p >= np) return;
+  const uint32_t id = (uint32_t)keys[p];
+  const float4* V = w2o + 3 * (size_t)id;
+  float4* P = prox + 4 * (size_t)p;
+  P[0] = V[0];
+  P[1] = V[1];
+  P[2] = V[2];
+  P[3] = make_float4(__uint_as_float(id), 0.0f, 0.0f, 0.0f);
+  lbox[2 * (size_t)p] = box[2 * (size_t)id];
+  lbox[2 * (size_t)p + 1] = box[2 * (size_t)id + 1];
+}
+
+__device__ __forceinline__ int kdelta(const unsigned long long* __restrict__ k, int np, int i, int j) {
+  if (j < 0 || j >= np) return -1;
+  return __clzll((long long)(k[i] ^ k[j])); /* keys are unique: the xor is never 0 */
+}
+
+/* inner node i of the radix tree over np >= 2 sorted unique keys */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_karras(const unsigned long long* __restrict__ k, int np,
+                                                        BNode* __restrict__ bn, uint32_t* __restrict__ lparent) {
+  const int i = (int)(blockIdx.x * BUILD_BLOCK + threadIdx.x);
+  if (i >= np - 1) return;
+  const int d = kdelta(k, np, i, i + 1) > kdelta(k, np, i, i - 1) ? 1 : -1;
+  const int dmin = kdelta(k, np, i, i - d);
+  int lmax = 2;
+  while (kdelta(k, np, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (kdelta(k, np, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = kdelta(k, np, i, j);
+  int s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (kdelta(k, np, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? d : 0);
+  const int lo = min(i, j), hi = max(i, j);
+  const uint32_t left = lo == gamma ? (BIN_LEAF | (uint32_t)gamma) : (uint32_t)gamma;
+  const uint32_t right = hi == gamma + 1 ? (BIN_LEAF | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+  bn[i].left = left;
+  bn[i].right = right;
+  bn[i].first = (uint32_t)lo;
+  bn[i].count = (uint32_t)(hi - lo + 1);
+  if (left & BIN_LEAF) lparent[gamma] = (uint32_t)i; else bn[gamma].parent = (uint32_t)i;
+  if (right & BIN_LEAF) lparent[gamma + 1] = (uint32_t)i; else bn[gamma + 1].parent = (uint32_t)i;
+  if (i == 0) bn[0].parent = BIN_NONE;
+}
+
+__device__ __forceinline__ void child_data(const BNode* bn, const float4* lbox, uint32_t ref, float* lo, float* hi,
+                                           double* cost) {
+  if (ref & BIN_LEAF) {
+    const uint32_t p = ref & ~BIN_LEAF;
+    const float4 a = lbox[2 * (size_t)p], b = lbox[2 * (size_t)p + 1];
+    lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+    hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+    *cost = GT_SAH_ISECT * box_area(lo, hi);
+  } else {
+    const BNode& c = bn[ref];
+    for (int j = 0; j < 3; ++j) { lo[j] = c.lo[j]; hi[j] = c.hi[j]; }
+    *cost = c.cost;
+  }
+}
+
+/* one thread per leaf climbs while it is the second child to arrive */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_refit(uint32_t np, BNode* __restrict__ bn,
+                                                       const float4* __restrict__ lbox,
+                                                       const uint32_t* __restrict__ lparent, uint32_t* visit) {
+  const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  uint32_t node = lparent[p];
+  while (node != BIN_NONE) {
+    __threadfence(); /* release this thread's writes below node */
+    if (atomicAdd(visit + node, 1u) == 0u) return;
+    __threadfence(); /* acquire the sibling's */
+    BNode* N = bn + node;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    double cl, cr;
+    child_data(bn, lbox, N->left, llo, lhi, &cl);
+    child_data(bn, lbox, N->right, rlo, rhi, &cr);
+    float lo[3], hi[3];
+    for (int j = 0; j < 3; ++j) {
+      lo[j] = fminf(llo[j], rlo[j]);
+      hi[j] = fmaxf(lhi[j], rhi[j]);
+      N->lo[j] = lo[j];
+      N->hi[j] = hi[j];
+    }
+    const double A = box_area(lo, hi);
+    const double split = 1.0 * A + cl + cr;
+    const double leafc = GT_SAH_ISECT * (double)N->count * A;
+    const bool leaf = N->count <= GT_LEAF_MAX && leafc <= split;
+    N->cost = leaf ? leafc : split;
+    N->leaf = leaf ? 1u : 0u;
+    node = N->parent;
+  }
+}
+
+__device__ __forceinline__ bool openable(const BNode* bn, uint32_t ref) {
+  return !(ref & BIN_LEAF) && bn[ref].leaf == 0u;
+}
+
+/* one 4-wide node per work item; cap = capacity of nodes[] and out[] */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restrict__ in, uint32_t n_in,
+                                                          Front* __restrict__ out, unsigned int* ctl,
+                                                          const BNode* __restrict__ bn,
+                                                          const float4* __restrict__ lbox,
+                                                          gt_node4* __restrict__ nodes, uint32_t cap) {
+  const uint32_t t = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (t >= n_in) return;
+  const Front e = in[t];
+  uint32_t ch[4];
+  int m;
+  if (!openable(bn, e.bref)) {
+    ch[0] = e.bref;
+    m = 1;
+  } else {
+    ch[0] = bn[e.bref].left;
+    ch[1] = bn[e.bref].right;
+    m = 2;
+    while (m < 4) {
+      int best = -1;
+      double best_area = -1.0;
+      for (int c = 0; c < m; ++c) {
+        if (!openable(bn, ch[c])) continue;
+        const double a = box_area(bn[ch[c]].lo, bn[ch[c]].hi);
+        if (a > best_area) { best_area = a; best = c; }
+      }
+      if (best < 0) break;
+      const uint32_t o = ch[best];
+      for (int c = m; c > best + 1; --c) ch[c] = ch[c - 1];
+      ch[best] = bn[o].left;
+      ch[best + 1] = bn[o].right;
+      ++m;
+    }
+  }
+  gt_node4 nd;
+  for (int c = 0; c < 4; ++c) {
+    nd.lo_x[c] = nd.lo_y[c] = nd.lo_z[c] = INFINITY; /* empty slots never hit */
+    nd.hi_x[c] = nd.hi_y[c] = nd.hi_z[c] = -INFINITY;
+    nd.child[c] = GT_NODE_EMPTY;
+    nd._pad[c] = 0u;
+  }
+  for (int c = 0; c < m; ++c) {
+    const uint32_t r = ch[c];
+    float lo[3], hi[3];
+    uint32_t ref;
+    if (r & BIN_LEAF) {
+      const uint32_t p = r & ~BIN_LEAF;
+      const float4 a = lbox[2 * (size_t)p], b = lbox[2 * (size_t)p + 1];
+      lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+      hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+      ref = dev_leaf_ref(p, 1u);
+    } else {
+      const BNode& b = bn[r];
+      for (int j = 0; j < 3; ++j) { lo[j] = b.lo[j]; hi[j] = b.hi[j]; }
+      if (b.leaf) {
+        ref = dev_leaf_ref(b.first, b.count);
+      } else {
+        const uint32_t j = atomicAdd(ctl + CTL_NODES, 1u);
+        const uint32_t slot = atomicAdd(ctl + CTL_OUT, 1u);
+        if (j < cap && slot < cap) {
+          out[slot] = Front{r, j};
+          ref = j;
+        } else {
+          atomicOr(ctl + CTL_OVF, 1u);
+          ref = GT_NODE_EMPTY;
+        }
+      }
+    }
+    nd.lo_x[c] = lo[0]; nd.hi_x[c] = hi[0];
+    nd.lo_y[c] = lo[1]; nd.hi_y[c] = hi[1];
+    nd.lo_z[c] = lo[2]; nd.hi_z[c] = hi[2];
+    nd.child[c] = ref;
+  }
+  if (e.node4 < cap) nodes[e.node4] = nd;
+}
+
+/* worst-case stack of the subtree: max(m, (m - 1) + max need of inner children) */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_need(const gt_node4* __restrict__ nodes, uint32_t first,
+                                                      uint32_t last, uint32_t* __restrict__ need) {
+  const uint32_t j = first + blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (j >= last) return;
+  const gt_node4 nd = nodes[j];
+  uint32_t m = 0, cmax = 0;
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t r = nd.child[c];
+    if (r == GT_NODE_EMPTY) continue;
+    ++m;
+    if (!(r & GT_LEAF_BIT)) cmax = max(cmax, need[r]);
+  }
+  need[j] = max(m, (m - 1u) + cmax);
+}
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + BUILD_BLOCK - 1) / BUILD_BLOCK); }
+
+}  // namespace
+
+struct gt_devbuild {
+  DevBuf o2w, w2o, box, keys_a, keys_b, sort_tmp, lbox, bnode, lparent, visit, front_a, front_b, need, ctl;
+  unsigned int* host_ctl = nullptr; /* pinned, CTL_WORDS */
+  ~gt_devbuild() {
+    if (host_ctl) (void)hipHostFree(host_ctl);
+  }
+};
+
+void gt_devbuild_destroy(gt_devbuild* w) { delete w; }
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      gt_set_error("%s failed: %s", #expr, hipGetErrorString(e_));                 \
+      return GT_E_HIP;                                                             \
+    }                                                                              \
+  } while (0)
+#define GT_TRY(expr)                \
+  do {                              \
+    const int rc_ = (expr);         \
+    if (rc_ != GT_OK) return rc_;   \
+  } while (0)
+
+static int read_ctl(gt_devbuild* w, hipStream_t st) {
+  HIP_TRY(hipMemcpyAsync(w->host_ctl, w->ctl.p, CTL_WORDS * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return GT_OK;
+}
+
+int gt_device_build(gt_scene* s, hipStream_t st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!s->db) {
+    s->db = new (std::nothrow) gt_devbuild();
+    if (!s->db) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  }
+  gt_devbuild* w = s->db;
+  if (!w->host_ctl) HIP_TRY(hipHostMalloc((void**)&w->host_ctl, CTL_WORDS * sizeof(unsigned int)));
+  const uint32_t n = s->n;
+  GT_TRY(gt_scene_static_upload(s));
+  GT_TRY(gt_devbuf_reserve(s->geom, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(s->sh, (size_t)n * 192));
+  GT_TRY(gt_devbuf_reserve(w->o2w, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(w->w2o, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(w->box, (size_t)n * 32));
+  GT_TRY(gt_devbuf_reserve(w->keys_a, (size_t)n * 8));
+  GT_TRY(gt_devbuf_reserve(w->keys_b, (size_t)n * 8));
+  GT_TRY(gt_devbuf_reserve(w->ctl, CTL_WORDS * sizeof(unsigned int)));
+  unsigned int* ctl = (unsigned int*)w->ctl.p;
+  k_init<<<1, 64, 0, st>>>(ctl);
+  HIP_TRY(hipGetLastError());
+  if (n > 0) {
+    k_prep<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float*)s->raw_dev.p, n, s->alpha_min, (float*)s->geom.p,
+                                                  (float*)s->sh.p, (float*)w->o2w.p, (float*)w->w2o.p,
+                                                  (float4*)w->box.p, ctl);
+    HIP_TRY(hipGetLastError());
+    k_keys<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float4*)w->box.p, n, ctl,
+                                                  (unsigned long long*)w->keys_a.p);
+    HIP_TRY(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIP_TRY(rocprim::radix_sort_keys(nullptr, tmp_bytes, (unsigned long long*)w->keys_a.p,
+                                     (unsigned long long*)w->keys_b.p, n, 0, 64, st));
+    GT_TRY(gt_devbuf_reserve(w->sort_tmp, tmp_bytes));
+    HIP_TRY(rocprim::radix_sort_keys(w->sort_tmp.p, tmp_bytes, (unsigned long long*)w->keys_a.p,
+                                     (unsigned long long*)w->keys_b.p, n, 0, 64, st));
+  }
+  GT_TRY(read_ctl(w, st));
+  const uint32_t np = w->host_ctl[CTL_NPROX];
+  if (np > n || np > GT_MAX_PROXIES) {
+    gt_set_error("device build: %u proxies for %u Gaussians (limit %u)", np, n, GT_MAX_PROXIES);
+    return GT_E_LIMIT;
+  }
+  /* a 4-wide node has >= 2 children (the root aside) and >= 4 unless all its
+   * children are leaves, so nodes <= 2 np / 3 + 1 */
+  const uint32_t cap = np ? (uint32_t)((2ull * np) / 3ull + 2ull) : 1u;
+  GT_TRY(gt_devbuf_reserve(s->prox, (size_t)np * 64));
+  GT_TRY(gt_devbuf_reserve(s->nodes, (size_t)cap * sizeof(gt_node4)));
+  uint32_t root = GT_NODE_EMPTY, n_nodes = 0, need = 0, depth = 0;
+  if (np > 0) {
+    GT_TRY(gt_devbuf_reserve(w->lbox, (size_t)np * 32));
+    GT_TRY(gt_devbuf_reserve(w->bnode, (size_t)np * sizeof(BNode)));
+    GT_TRY(gt_devbuf_reserve(w->lparent, (size_t)np * 4));
+    GT_TRY(gt_devbuf_reserve(w->visit, (size_t)np * 4));
+    GT_TRY(gt_devbuf_reserve(w->front_a, (size_t)cap * sizeof(Front)));
+    GT_TRY(gt_devbuf_reserve(w->front_b, (size_t)cap * sizeof(Front)));
+    GT_TRY(gt_devbuf_reserve(w->need, (size_t)cap * 4));
+    BNode* bn = (BNode*)w->bnode.p;
+    const float4* lbox = (const float4*)w->lbox.p;
+    k_gather<<<blocks_for(np), BUILD_BLOCK, 0, st>>>((const unsigned long long*)w->keys_b.p, np,
+                                                     (const float4*)w->w2o.p, (const float4*)w->box.p,
+                                                     (float4*)s->prox.p, (float4*)w->lbox.p, (Front*)w->front_a.p);
+    HIP_TRY(hipGetLastError());
+    if (np >= 2) {
+      k_fill<<<blocks_for(np), BUILD_BLOCK, 0, st>>>((uint32_t*)w->visit.p, np, 0u);
+      HIP_TRY(hipGetLastError());
+      k_karras<<<blocks_for(np - 1), BUILD_BLOCK, 0, st>>>((const unsigned long long*)w->keys_b.p, (int)np, bn,
+                                                           (uint32_t*)w->lparent.p);
+      HIP_TRY(hipGetLastError());
+      k_refit<<<blocks_for(np), BUILD_BLOCK, 0, st>>>(np, bn, lbox, (const uint32_t*)w->lparent.p,
+                                                      (uint32_t*)w->visit.p);
+      HIP_TRY(hipGetLastError());
+    }
+    /* top-down collapse, one launch per level; level L's nodes are
+     * [level[L], level[L+1]) because a level's nodes are all allocated by
+     * the launch of the level above */
+    std::vector<uint32_t> level{0u, 1u};
+    Front* fin = (Front*)w->front_a.p;
+    Front* fout = (Front*)w->front_b.p;
+    uint32_t n_in = 1;
+    while (n_in > 0) {
+      if (level.size() > GT_STACK_MAX + 2) {
+        gt_set_error("device build: BVH deeper than %u levels", GT_STACK_MAX);
+        return GT_E_LIMIT;
+      }
+      k_fill<<<1, BUILD_BLOCK, 0, st>>>(ctl + CTL_OUT, 1u, 0u);
+      HIP_TRY(hipGetLastError());
+      k_collapse<<<blocks_for(n_in), BUILD_BLOCK, 0, st>>>(fin, n_in, fout, ctl, bn, lbox, (gt_node4*)s->nodes.p, cap);
+      HIP_TRY(hipGetLastError());
+      GT_TRY(read_ctl(w, st));
+      if (w->host_ctl[CTL_OVF]) {
+        gt_set_error("device build: more than %u BVH nodes", cap);
+        return GT_E_LIMIT;
+      }
+      n_in = w->host_ctl[CTL_OUT];
+      level.push_back(w->host_ctl[CTL_NODES]);
+      std::swap(fin, fout);
+    }
+    n_nodes = level.back();
+    depth = (uint32_t)level.size() - 2u;
+    uint32_t* needp = (uint32_t*)w->need.p;
+    for (size_t L = level.size() - 2; L-- > 0;) {
+      const uint32_t a = level[L], b = level[L + 1];
+      if (b > a) {
+        k_need<<<blocks_for(b - a), BUILD_BLOCK, 0, st>>>((const gt_node4*)s->nodes.p, a, b, needp);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+    HIP_TRY(hipMemcpyAsync(w->host_ctl, needp, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    need = w->host_ctl[0];
+    root = 0;
+  }
+  if (need > GT_STACK_MAX) {
+    gt_set_error("BVH needs a traversal stack of %u entries (limit %u)", need, GT_STACK_MAX);
+    return GT_E_LIMIT;
+  }
+  s->dev.geom = (const float*)s->geom.p;
+  s->dev.sh = (const float*)s->sh.p;
+  s->dev.prox = (const float*)s->prox.p;
+  s->dev.nodes = (const gt_node4*)s->nodes.p;
+  s->dev.faces = (const float*)s->faces.p;
+  s->dev.root = root;
+  s->dev.n = n;
+  s->dev.n_prox = np;
+  s->dev.n_nodes = n_nodes;
+  s->n_nodes = n_nodes;
+  s->max_stack = need;
+  s->depth = depth;
+  s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GT_OK;
+}
+
+int gt_device_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w, float* w2o) {
+  const uint32_t n = s->n, np = s->dev.n_prox;
+  HIP_TRY(hipDeviceSynchronize());
+  if (packed) {
+    std::vector<float> geom((size_t)n * 12), sh((size_t)n * 48);
+    if (n) {
+      HIP_TRY(hipMemcpy(geom.data(), s->geom.p, geom.size() * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(sh.data(), s->sh.p, sh.size() * 4, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      float* o = packed + (size_t)i * 80;
+      const float* g = &geom[(size_t)i * 12];
+      std::memset(o, 0, 80 * sizeof(float));
+      o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+      o[4] = g[4]; o[5] = g[5]; o[6] = g[6];
+      o[8] = g[8]; o[9] = g[9]; o[10] = g[10]; o[11] = g[11];
+      o[12] = g[3];
+      for (int j = 0; j < 16; ++j)
+        for (int c = 0; c < 3; ++c) o[16 + 4 * j + c] = sh[(size_t)i * 48 + 3 * j + c];
+    }
+  }
+  if (proxy_ids || o2w || w2o) {
+    std::vector<float> prox((size_t)np * 16), m((size_t)n * 12);
+    if (np) HIP_TRY(hipMemcpy(prox.data(), s->prox.p, prox.size() * 4, hipMemcpyDeviceToHost));
+    if (n && o2w) HIP_TRY(hipMemcpy(m.data(), s->db->o2w.p, m.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<std::pair<uint32_t, uint32_t>> order(np); /* (gid, sorted position) */
+    for (uint32_t p = 0; p < np; ++p) {
+      uint32_t gid;
+      std::memcpy(&gid, &prox[(size_t)p * 16 + 12], 4);
+      order[p] = {gid, p};
+    }
+    std::sort(order.begin(), order.end());
+    for (uint32_t q = 0; q < np; ++q) {
+      const uint32_t gid = order[q].first, p = order[q].second;
+      if (proxy_ids) proxy_ids[q] = gid;
+      if (w2o) std::memcpy(w2o + (size_t)q * 12, &prox[(size_t)p * 16], 12 * sizeof(float));
+      if (o2w) std::memcpy(o2w + (size_t)q * 12, &m[(size_t)gid * 12], 12 * sizeof(float));
+    }
+  }
+  return GT_OK;
+}

@@ -27,6 +27,10 @@
 #define GT_LEAF_FIRST_MASK 0x0fffffffu
 #define GT_MAX_PROXIES GT_LEAF_FIRST_MASK
 #define GT_STACK_MAX 96               /* traversal stack entries per ray */
+/* SAH cost of one proxy test relative to one box test (host and device builders) */
+#ifndef GT_SAH_ISECT
+#define GT_SAH_ISECT 2.0
+#endif
 
 static inline uint32_t gt_leaf_ref(uint32_t first, uint32_t count) {
   return GT_LEAF_BIT | ((count - 1u) << GT_LEAF_COUNT_SHIFT) | first;

@@ -26,42 +26,8 @@
 #include "../../include/gt.h"
 #include "../../include/gt_fp32.h"
 #include "gt_internal.h"
-
-namespace {
-
-struct HostScene {
-  uint32_t n = 0;
-  std::vector<float> raw;      // n x 59 (empty for packed input)
-  std::vector<double> act;     // n x 8: opacity, scale[3], quat[4] (JS doubles)
-  std::vector<float> packed;   // n x 80
-  // proxies, in original order
-  std::vector<uint32_t> proxy_ids;
-  std::vector<float> o2w, w2o;  // n' x 12
-};
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
-}  // namespace
-
-struct gt_scene {
-  HostScene h;
-  double alpha_min = 0.0;
-  // device
-  int device = 0;
-  DevBuf geom, sh, prox, nodes, faces;
-  DevBuf ctl; /* work counter of the persistent renderer */
-  DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
-  DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
-  uint32_t rc_cap = 0; /* checkpoint capacity the last recompute needed */
-  gt_tuning tune{};
-  gt_dev_scene dev{};
-  uint32_t n_nodes = 0, max_stack = 0, depth = 0;
-  double build_ms = 0.0;
-};
+#include "gt_prep.h"
+#include "gt_scene_impl.h"
 
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
@@ -77,23 +43,15 @@ static void activate_raw(HostScene& h) {
   h.act.assign((size_t)h.n * 8, 0.0);
   h.packed.assign((size_t)h.n * 80, 0.0f);
   for (uint32_t i = 0; i < h.n; ++i) {
-    const float* r = &h.raw[(size_t)i * GT_RAW_FLOATS];
+    float g[12], sh[48];
+    gt_prep_activate(&h.raw[(size_t)i * GT_RAW_FLOATS], &h.act[(size_t)i * 8], g, sh);
     float* o = &h.packed[(size_t)i * 80];
-    double* a = &h.act[(size_t)i * 8];
-    double sc0 = std::exp((double)r[52]), sc1 = std::exp((double)r[53]), sc2 = std::exp((double)r[54]);
-    double q0 = r[55], q1 = r[56], q2 = r[57], q3 = r[58];
-    double norm = std::sqrt(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
-    q0 /= norm; q1 /= norm; q2 /= norm; q3 /= norm;
-    double op = 1.0 / (1.0 + std::exp(-(double)r[51]));
-    o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
-    o[4] = (float)sc0; o[5] = (float)sc1; o[6] = (float)sc2;
-    o[8] = (float)q0; o[9] = (float)q1; o[10] = (float)q2; o[11] = (float)q3;
-    o[12] = (float)op;
-    for (int c = 0; c < 3; ++c) o[16 + c] = r[3 + c];
-    for (int j = 0; j < 15; ++j)
-      for (int c = 0; c < 3; ++c) o[16 + 4 * (1 + j) + c] = r[6 + c * 15 + j];
-    a[0] = op; a[1] = sc0; a[2] = sc1; a[3] = sc2;
-    a[4] = q0; a[5] = q1; a[6] = q2; a[7] = q3;
+    o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+    o[4] = g[4]; o[5] = g[5]; o[6] = g[6];
+    o[8] = g[8]; o[9] = g[9]; o[10] = g[10]; o[11] = g[11];
+    o[12] = g[3];
+    for (int j = 0; j < 16; ++j)
+      for (int c = 0; c < 3; ++c) o[16 + 4 * j + c] = sh[3 * j + c];
   }
 }
 
@@ -109,53 +67,13 @@ static void activate_packed(HostScene& h) {
 }
 
 /* ---------------- proxies (src/main.ts:73-94, webrtx/bvh/src/lib.rs:501-505) ---------------- */
-/* Rotation of wgpu-matrix mat3.fromQuat([x, y, z, w]) evaluated in double
- * and stored as f32; R[c][r] = column c, row r. */
-static void quat_to_mat_f32(double x, double y, double z, double w, float R[3][3]) {
-  const double x2 = x + x, y2 = y + y, z2 = z + z;
-  const double xx = x * x2, yx = y * x2, yy = y * y2;
-  const double zx = z * x2, zy = z * y2, zz = z * z2;
-  const double wx = w * x2, wy = w * y2, wz = w * z2;
-  R[0][0] = (float)(1 - yy - zz); R[0][1] = (float)(yx + wz); R[0][2] = (float)(zx - wy);
-  R[1][0] = (float)(yx - wz); R[1][1] = (float)(1 - xx - zz); R[1][2] = (float)(zy + wx);
-  R[2][0] = (float)(zx + wy); R[2][1] = (float)(zy - wx); R[2][2] = (float)(1 - xx - yy);
-}
-
-/* glam Affine3A::inverse (f32, scalar build): rows of the inverse 3x3 are
- * (y x z, z x x, x x y) / det(z . (x x y)), translation = -(inv * t). */
-static void invert_affine_f32(const float* m, float* out) {
-  const gt_f3 cx = gt_f3_make(m[0], m[1], m[2]);
-  const gt_f3 cy = gt_f3_make(m[3], m[4], m[5]);
-  const gt_f3 cz = gt_f3_make(m[6], m[7], m[8]);
-  const gt_f3 row0 = gt_cross(cy, cz), row1 = gt_cross(cz, cx), row2 = gt_cross(cx, cy);
-  const float inv_det = 1.0f / gt_dot(cz, row2);
-  const gt_f3 a = gt_scale(inv_det, row0), b = gt_scale(inv_det, row1), c = gt_scale(inv_det, row2);
-  const gt_f3 k0 = gt_f3_make(a.x, b.x, c.x), k1 = gt_f3_make(a.y, b.y, c.y), k2 = gt_f3_make(a.z, b.z, c.z);
-  const gt_f3 it = gt_add(gt_add(gt_scale(m[9], k0), gt_scale(m[10], k1)), gt_scale(m[11], k2));
-  const float v[12] = {k0.x, k0.y, k0.z, k1.x, k1.y, k1.z, k2.x, k2.y, k2.z, -it.x, -it.y, -it.z};
-  std::memcpy(out, v, sizeof(v));
-}
-
 static void build_proxies(HostScene& h, double alpha_min) {
   h.proxy_ids.clear();
   h.o2w.clear();
   h.w2o.clear();
   for (uint32_t i = 0; i < h.n; ++i) {
-    const double* a = &h.act[(size_t)i * 8];
-    const double opacity = a[0];
-    if (!(opacity > alpha_min)) continue;
-    const double s = std::sqrt(2 * std::log(opacity / alpha_min));
-    float S[3];
-    for (int j = 0; j < 3; ++j) S[j] = (float)(s * a[1 + j]);
-    float R[3][3];
-    quat_to_mat_f32(a[5], a[6], a[7], a[4], R);
-    float m[12];
-    for (int c = 0; c < 3; ++c)
-      for (int r = 0; r < 3; ++r) m[c * 3 + r] = (float)((double)R[c][r] * (double)S[c]);
-    const float* pos = &h.packed[(size_t)i * 80];
-    m[9] = pos[0]; m[10] = pos[1]; m[11] = pos[2];
-    float inv[12];
-    invert_affine_f32(m, inv);
+    float m[12], inv[12];
+    if (!gt_prep_proxy(&h.act[(size_t)i * 8], &h.packed[(size_t)i * 80], alpha_min, m, inv)) continue;
     h.proxy_ids.push_back(i);
     h.o2w.insert(h.o2w.end(), m, m + 12);
     h.w2o.insert(h.w2o.end(), inv, inv + 12);
@@ -185,35 +103,12 @@ struct B2Node {
   uint32_t first = 0, count = 0;
 };
 
-/* World box of a proxy: the 12 icosahedron vertices through O2W in double,
- * padded by far more than the FP32 error of the object-space triangle test,
- * then rounded outward to f32.  Culling with these boxes never drops a hit. */
 static Box proxy_box(const float* m) {
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int v = 0; v < 12; ++v) {
-    const gt_f3 p = gt_ico_vertex(v);
-    for (int r = 0; r < 3; ++r) {
-      const double w = (double)m[r] * p.x + (double)m[3 + r] * p.y + (double)m[6 + r] * p.z + (double)m[9 + r];
-      lo[r] = std::min(lo[r], w);
-      hi[r] = std::max(hi[r], w);
-    }
-  }
   Box b;
-  for (int r = 0; r < 3; ++r) {
-    const double pad = 1e-3 * (hi[r] - lo[r]) + 2e-5 * (1.0 + std::fabs(lo[r]) + std::fabs(hi[r]));
-    float l = (float)(lo[r] - pad), u = (float)(hi[r] + pad);
-    if ((double)l > lo[r] - pad) l = std::nextafter(l, -INFINITY);
-    if ((double)u < hi[r] + pad) u = std::nextafter(u, INFINITY);
-    b.lo[r] = l;
-    b.hi[r] = u;
-  }
+  gt_prep_box(m, b.lo, b.hi);
   return b;
 }
 
-/* SAH cost of one proxy test relative to one box test */
-#ifndef GT_SAH_ISECT
-#define GT_SAH_ISECT 2.0
-#endif
 
 struct Builder {
   const std::vector<Box>& boxes;
@@ -436,6 +331,34 @@ static int upload(DevBuf& b, const T* src, size_t count) {
 }
 
 
+int gt_devbuf_reserve(DevBuf& b, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (b.p && b.bytes >= bytes) return GT_OK;
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+  const hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    gt_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? GT_E_NOMEM : GT_E_HIP;
+  }
+  b.bytes = bytes;
+  return GT_OK;
+}
+
+int gt_scene_static_upload(gt_scene* s) {
+  int rc;
+  if (!s->faces.p) {
+    float faces[12 * GT_NUM_FACES];
+    face_table(faces);
+    if ((rc = upload(s->faces, faces, 12 * GT_NUM_FACES)) != GT_OK) return rc;
+  }
+  if (!s->ctl.p) {
+    const uint32_t zero[64] = {0};
+    if ((rc = upload(s->ctl, zero, 64)) != GT_OK) return rc;
+  }
+  return GT_OK;
+}
+
 static int build_and_upload(gt_scene* s) {
   const auto t0 = std::chrono::steady_clock::now();
   HostScene& h = s->h;
@@ -489,24 +412,19 @@ static int build_and_upload(gt_scene* s) {
     for (int j = 0; j < 16; ++j)
       for (int c = 0; c < 3; ++c) sh[(size_t)i * 48 + 3 * j + c] = o[16 + 4 * j + c];
   }
-  float faces[12 * GT_NUM_FACES];
-  face_table(faces);
   int rc;
   if ((rc = upload(s->geom, geom.data(), geom.size())) != GT_OK) return rc;
   if ((rc = upload(s->sh, sh.data(), sh.size())) != GT_OK) return rc;
   if ((rc = upload(s->prox, prox.data(), prox.size())) != GT_OK) return rc;
   if ((rc = upload(s->nodes, nodes.data(), nodes.size())) != GT_OK) return rc;
-  if ((rc = upload(s->faces, faces, 12 * GT_NUM_FACES)) != GT_OK) return rc;
-  if (!s->ctl.p) {
-    const uint32_t zero[64] = {0};
-    if ((rc = upload(s->ctl, zero, 64)) != GT_OK) return rc;
-  }
+  if ((rc = gt_scene_static_upload(s)) != GT_OK) return rc;
   s->dev.geom = (const float*)s->geom.p;
   s->dev.sh = (const float*)s->sh.p;
   s->dev.prox = (const float*)s->prox.p;
   s->dev.nodes = (const gt_node4*)s->nodes.p;
   s->dev.faces = (const float*)s->faces.p;
   s->dev.root = root;
+  s->n = h.n;
   s->dev.n = h.n;
   s->dev.n_prox = np;
   s->dev.n_nodes = (uint32_t)nodes.size();
@@ -540,6 +458,7 @@ extern "C" int gt_scene_create_raw(const float* raw, uint32_t n, double alpha_mi
   if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
   try {
     s->h.n = n;
+    s->n = n;
     s->h.raw.assign(raw, raw + (size_t)n * GT_RAW_FLOATS);
     s->alpha_min = alpha_min;
     activate_raw(s->h);
@@ -562,6 +481,7 @@ extern "C" int gt_scene_create_packed(const void* records, uint32_t n, double al
   if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
   try {
     s->h.n = n;
+    s->n = n;
     const float* f = (const float*)records;
     s->h.packed.assign(f, f + (size_t)n * 80);
     s->alpha_min = alpha_min;
@@ -580,6 +500,7 @@ extern "C" int gt_scene_set_alpha_min(gt_scene* s, double alpha_min) {
     return GT_E_INVALID;
   }
   s->alpha_min = alpha_min;
+  if (s->device_built) return gt_device_build(s, nullptr);
   try {
     return build_and_upload(s);
   } catch (const std::bad_alloc&) {
@@ -590,15 +511,63 @@ extern "C" int gt_scene_set_alpha_min(gt_scene* s, double alpha_min) {
 
 extern "C" void gt_scene_destroy(gt_scene* s) { delete s; }
 
+/* ---------------- device build (GPU LBVH) ---------------- */
+static int device_rebuild(gt_scene* s, const float* raw_dev, void* stream) {
+  const size_t bytes = (size_t)s->n * GT_RAW_FLOATS * sizeof(float);
+  int rc = gt_devbuf_reserve(s->raw_dev, bytes);
+  if (rc != GT_OK) return rc;
+  if (bytes) HIP_TRY(hipMemcpyAsync(s->raw_dev.p, raw_dev, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return gt_device_build(s, (hipStream_t)stream);
+}
+
+extern "C" int gt_scene_create_device(const float* raw_dev, uint32_t n, double alpha_min, void* stream,
+                                      gt_scene** out) {
+  if (!out) { gt_set_error("gt_scene_create_device: out is NULL"); return GT_E_INVALID; }
+  *out = nullptr;
+  if (n && !raw_dev) { gt_set_error("gt_scene_create_device: raw_dev is NULL"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_scene_create_device: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  if (n > GT_MAX_PROXIES) {
+    gt_set_error("gt_scene_create_device: %u Gaussians exceed the limit %u", n, GT_MAX_PROXIES);
+    return GT_E_LIMIT;
+  }
+  std::unique_ptr<gt_scene> s(new (std::nothrow) gt_scene());
+  if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    gt_set_error("no HIP device available");
+    return GT_E_HIP;
+  }
+  s->n = n;
+  s->device_built = true;
+  s->alpha_min = alpha_min;
+  const int rc = device_rebuild(s.get(), raw_dev, stream);
+  if (rc != GT_OK) return rc;
+  *out = s.release();
+  return GT_OK;
+}
+
+extern "C" int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* stream) {
+  if (!s) { gt_set_error("gt_scene_update_device: scene is NULL"); return GT_E_INVALID; }
+  if (!s->device_built) {
+    gt_set_error("gt_scene_update_device: the scene was not created by gt_scene_create_device");
+    return GT_E_INVALID;
+  }
+  if (s->n && !raw_dev) { gt_set_error("gt_scene_update_device: raw_dev is NULL"); return GT_E_INVALID; }
+  return device_rebuild(s, raw_dev, stream);
+}
+
 extern "C" int gt_scene_get_info(const gt_scene* s, gt_scene_info* info) {
   if (!s || !info) { gt_set_error("gt_scene_get_info: NULL argument"); return GT_E_INVALID; }
   std::memset(info, 0, sizeof(*info));
-  info->n_gaussians = s->h.n;
+  info->n_gaussians = s->n;
   info->n_proxies = s->dev.n_prox;
   info->n_nodes = s->n_nodes;
   info->max_stack = s->max_stack;
   info->bvh_depth = s->depth;
-  info->device_bytes = s->geom.bytes + s->sh.bytes + s->prox.bytes + s->nodes.bytes + s->faces.bytes;
+  info->device_bytes = s->geom.bytes + s->sh.bytes + s->prox.bytes + s->nodes.bytes + s->faces.bytes +
+                      s->raw_dev.bytes;
   info->build_ms = s->build_ms;
   return GT_OK;
 }
@@ -606,6 +575,7 @@ extern "C" int gt_scene_get_info(const gt_scene* s, gt_scene_info* info) {
 extern "C" int gt_scene_get_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w,
                                       float* w2o) {
   if (!s) { gt_set_error("gt_scene_get_host_data: scene is NULL"); return GT_E_INVALID; }
+  if (s->device_built) return gt_device_host_data(s, packed, proxy_ids, o2w, w2o);
   const HostScene& h = s->h;
   if (packed) std::memcpy(packed, h.packed.data(), h.packed.size() * sizeof(float));
   if (proxy_ids) std::memcpy(proxy_ids, h.proxy_ids.data(), h.proxy_ids.size() * sizeof(uint32_t));

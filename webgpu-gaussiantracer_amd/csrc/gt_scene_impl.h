@@ -1,0 +1,70 @@
+/*
+ * gt_scene_impl.h - the scene handle behind gt_scene* (C++ only), shared by
+ * the host builder and render wrappers (gt_scene.cpp) and the device
+ * builder (gt_build.hip).
+ */
+#ifndef GT_SCENE_IMPL_H
+#define GT_SCENE_IMPL_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "gt_internal.h"
+
+struct HostScene {
+  uint32_t n = 0;
+  std::vector<float> raw;      // n x 59 (empty for packed input)
+  std::vector<double> act;     // n x 8: opacity, scale[3], quat[4] (JS doubles)
+  std::vector<float> packed;   // n x 80
+  // proxies, in original order
+  std::vector<uint32_t> proxy_ids;
+  std::vector<float> o2w, w2o;  // n' x 12
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+/* Device-build workspace (gt_build.hip): temporaries kept between rebuilds. */
+struct gt_devbuild;
+void gt_devbuild_destroy(gt_devbuild* w);
+
+struct gt_scene {
+  HostScene h;                /* host-built scenes only */
+  uint32_t n = 0;             /* Gaussians */
+  bool device_built = false;  /* built on the GPU from raw_dev (gt_scene_create_device) */
+  double alpha_min = 0.0;
+  // device
+  int device = 0;
+  DevBuf geom, sh, prox, nodes, faces;
+  DevBuf raw_dev; /* device-built scenes: copy of the n x 59 PLY-raw parameters */
+  gt_devbuild* db = nullptr;
+  DevBuf ctl; /* work counter of the persistent renderer */
+  DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
+  DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
+  uint32_t rc_cap = 0; /* checkpoint capacity the last recompute needed */
+  gt_tuning tune{};
+  gt_dev_scene dev{};
+  uint32_t n_nodes = 0, max_stack = 0, depth = 0;
+  double build_ms = 0.0;
+  ~gt_scene() { gt_devbuild_destroy(db); }
+};
+
+/* gt_scene.cpp: (re)allocates b to exactly max(bytes, 256) bytes unless it
+ * already holds at least bytes; GT_OK or an error code (message set). */
+int gt_devbuf_reserve(DevBuf& b, size_t bytes);
+/* uploads the icosahedron face table and the persistent renderer's counter once */
+int gt_scene_static_upload(gt_scene* s);
+
+/* gt_build.hip: builds every device array of s from s->raw_dev on the GPU */
+int gt_device_build(gt_scene* s, hipStream_t stream);
+/* gt_build.hip: host copies of a device-built scene (gt_scene_get_host_data) */
+int gt_device_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w, float* w2o);
+
+#endif

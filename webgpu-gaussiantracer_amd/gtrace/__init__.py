@@ -100,6 +100,8 @@ _lib.gt_camera_look_at.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C
 _lib.gt_scene_create_raw.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
 _lib.gt_scene_create_packed.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
 _lib.gt_scene_set_alpha_min.argtypes = [_P, C.c_double]
+_lib.gt_scene_create_device.argtypes = [_P, C.c_uint32, C.c_double, _P, C.POINTER(_P)]
+_lib.gt_scene_update_device.argtypes = [_P, _P, _P]
 _lib.gt_scene_destroy.argtypes = [_P]
 _lib.gt_scene_destroy.restype = None
 _lib.gt_scene_get_info.argtypes = [_P, C.POINTER(SceneInfo)]
@@ -127,7 +129,7 @@ _lib.gt_free.restype = None
 
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
-    "gt_scene_set_alpha_min", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
+    "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_update_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
     "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
@@ -251,11 +253,20 @@ def host_prepare(raw, alpha_min):
 
 
 class Scene:
-    """Acceleration container over the Gaussians (gaussianSceneToBvh, src/main.ts:30-111)."""
+    """Acceleration container over the Gaussians (gaussianSceneToBvh, src/main.ts:30-111).
 
-    def __init__(self, raw=None, alpha_min=0.2, packed=None):
+    raw / packed: host arrays, built on the host (binned SAH).  raw_dev: a
+    device pointer to n x 59 PLY-raw floats (e.g. a torch tensor's
+    data_ptr()), built on the GPU (gt_scene_create_device); update_device()
+    rebuilds it after the parameters change."""
+
+    def __init__(self, raw=None, alpha_min=0.2, packed=None, raw_dev=None, n=None, stream_ptr=None):
         self._h = C.c_void_p()
-        if raw is not None:
+        if raw_dev is not None:
+            if n is None:
+                raise ValueError("n is required with raw_dev")
+            _check(_lib.gt_scene_create_device(raw_dev, int(n), float(alpha_min), stream_ptr, C.byref(self._h)))
+        elif raw is not None:
             raw = np.ascontiguousarray(raw, dtype=np.float32)
             assert raw.ndim == 2 and raw.shape[1] == GT_RAW_FLOATS
             _check(_lib.gt_scene_create_raw(_ptr(raw), raw.shape[0], float(alpha_min), C.byref(self._h)))
@@ -282,6 +293,10 @@ class Scene:
 
     def set_alpha_min(self, alpha_min):
         _check(_lib.gt_scene_set_alpha_min(self._h, float(alpha_min)))
+
+    def update_device(self, raw_dev, stream_ptr=None):
+        """Rebuilds a device-built scene from updated device parameters (same n)."""
+        _check(_lib.gt_scene_update_device(self._h, raw_dev, stream_ptr))
 
     def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0):
         """Launch tuning (results do not depend on it); 0 = library default."""
