@@ -121,7 +121,7 @@ typedef struct gt_scene_info {
   uint32_t bvh_depth;
   uint32_t _pad;
   uint64_t device_bytes;    /* HBM held by the scene */
-  double build_ms;          /* host build time (activations, proxies, BVH) */
+  double build_ms;          /* last build time (activations, proxies, BVH; host or GPU) */
 } gt_scene_info;
 int gt_scene_get_info(const gt_scene* s, gt_scene_info* info);
 
@@ -230,6 +230,23 @@ int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height
 int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, uint32_t height, const float* dl_drgba,
                 const uint32_t* comp_ids, const uint32_t* n_comp, uint32_t comp_cap, float* grads,
                 unsigned int* overflow, uint64_t budget_bytes, void* hip_stream);
+
+/* ---- training (no counterpart in the reference, which has no optimizer;
+ * README.md:68-69 plans training) ----
+ * One Adam update (Kingma & Ba 2015, the operation order of PyTorch's
+ * torch.optim.Adam) in place on n x 59 PLY-raw parameters given their
+ * gradient (gt_backward's layout), with one learning rate per field
+ * (column; 0 keeps a field fixed).  m and v are the moment buffers (same
+ * shape, zero before step 1).  Device pointers; asynchronous on the stream.
+ * A training step is gt_render (with checkpoint) + dL/dRGBA + gt_backward
+ * per view, gt_adam_step, then gt_scene_update_device. */
+typedef struct gt_adam {
+  float lr[GT_RAW_FLOATS];
+  float beta1, beta2, eps;
+  uint32_t step; /* 1 for the first update (bias correction) */
+} gt_adam;
+int gt_adam_step(float* params, const float* grads, float* m, float* v, uint32_t n, const gt_adam* adam,
+                 void* hip_stream);
 
 /* ---- batched cameras (no counterpart in the reference, which renders one
  * camera per traceRays dispatch, pass_encoder.ts:35-65) ---- */

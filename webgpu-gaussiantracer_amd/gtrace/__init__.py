@@ -81,6 +81,11 @@ class AutotuneResult(C.Structure):
                 ("_pad", C.c_uint32)]
 
 
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float * GT_RAW_FLOATS), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("step", C.c_uint32)]
+
+
 class GenParams(C.Structure):
     _fields_ = [
         ("n", C.c_uint32), ("n_clusters", C.c_uint32), ("box_half", C.c_float),
@@ -121,6 +126,7 @@ _lib.gt_autotune_batch.argtypes = [_P, _P, C.c_int, C.c_uint32, _P, C.POINTER(Au
 _lib.gt_backward.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P,
                              C.c_uint64, _P]
 _lib.gt_render.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, C.POINTER(Frame), C.c_int, _P]
+_lib.gt_adam_step.argtypes = [_P, _P, _P, _P, C.c_uint32, C.POINTER(Adam), _P]
 _lib.gt_generate_scene.argtypes = [C.POINTER(GenParams), _P]
 _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                              C.POINTER(C.c_uint32)]
@@ -130,7 +136,7 @@ _lib.gt_free.restype = None
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_update_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
+    "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
 
@@ -439,3 +445,13 @@ def backward(scene, u, width, height, dl_drgba_ptr, grads_ptr, comp_ids_ptr=None
     """Accumulates dLoss/draw (n x 59, device) for one frame; device pointers."""
     _check(_lib.gt_backward(scene.handle, C.byref(u), width, height, dl_drgba_ptr, comp_ids_ptr, n_comp_ptr,
                             comp_cap, grads_ptr, overflow_ptr, budget_bytes, stream_ptr))
+
+
+def adam_step(params_ptr, grads_ptr, m_ptr, v_ptr, n, lr, step, beta1=0.9, beta2=0.999, eps=1e-15, stream_ptr=None):
+    """One in-place Adam update of n x 59 device parameters; lr: 59 per-field rates."""
+    a = Adam()
+    lr = np.broadcast_to(np.asarray(lr, np.float32), (GT_RAW_FLOATS,))
+    for i in range(GT_RAW_FLOATS):
+        a.lr[i] = float(lr[i])
+    a.beta1, a.beta2, a.eps, a.step = beta1, beta2, eps, int(step)
+    _check(_lib.gt_adam_step(params_ptr, grads_ptr, m_ptr, v_ptr, int(n), C.byref(a), stream_ptr))

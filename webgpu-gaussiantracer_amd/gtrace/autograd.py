@@ -5,8 +5,9 @@
 raw is the (n, 59) PLY-raw parameter tensor (mean, f_dc, f_rest, opacity
 logit, log-scale, raw quaternion); rgba is the (H, W, 4) device image of
 gt_render.  The backward kernel returns dLoss/draw in the same layout.  The
-scene (proxies + BVH) is built from raw's values at each forward call, so
-the hit lists the adjoint replays are those of this forward.
+scene (proxies + BVH) is built from raw's values at each forward call (on the
+GPU when raw is a CUDA tensor, gt_scene_create_device), so the hit lists the
+adjoint replays are those of this forward.
 """
 import torch
 
@@ -19,9 +20,13 @@ class RenderFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, raw, u, width, height, alpha_min, mode):
         dev = torch.device("cuda", torch.cuda.current_device())
-        scene = Scene(raw.detach().to("cpu", torch.float32).numpy(), alpha_min=alpha_min)
-        rgba = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
         stream = torch.cuda.current_stream().cuda_stream
+        if raw.is_cuda:  # built on the GPU from the device parameters
+            r = raw.detach().to(torch.float32).contiguous()
+            scene = Scene(raw_dev=r.data_ptr(), n=r.shape[0], alpha_min=alpha_min, stream_ptr=stream)
+        else:
+            scene = Scene(raw.detach().to("cpu", torch.float32).numpy(), alpha_min=alpha_min)
+        rgba = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
         if mode == "checkpoint":
             cap = CHECKPOINT_CAP
             ncomp = torch.empty(height * width, dtype=torch.int32, device=dev)
