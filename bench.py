@@ -23,6 +23,10 @@ hipGraph, against the same cameras rendered one launch at a time.
 fwd_bwd_c4: the same forward + backward on 200k single-cluster Gaussians, 800x800.
 c5: 3M Gaussians, 16 views at 1280x720, forward + recompute-mode backward
 per view with gradients accumulated across views, peak device memory.
+device_build: the same 1M-Gaussian scene built on the GPU from device
+parameters (rebuild time) and the forward frame time on that tree.
+train_c4: one training step (forward with checkpoint, backward, Adam, GPU
+rebuild) on the 200k-Gaussian 800x800 workload.
 
 cpu_baseline: the oracle (the project's C restatement of the reference path,
 BVH variant, all host threads up to 16) on a seeded 1/3 sample of the rays.
@@ -326,6 +330,66 @@ def bench_c5(gt, torch, dev, sp, steps):
     }
 
 
+def bench_device_build(gt, torch, dev, sp, raw, u, W, H, steps):
+    """The same scene built on the GPU from device-resident parameters
+    (gt_scene_create_device: activations, proxies, Morton radix-tree BVH),
+    rebuild time (gt_scene_update_device) and the forward frame time on the
+    GPU-built tree."""
+    stream = torch.cuda.current_stream()
+    x = torch.from_numpy(raw).to(dev)
+    torch.cuda.synchronize()
+    scene = gt.Scene(raw_dev=x.data_ptr(), n=raw.shape[0], alpha_min=0.01, stream_ptr=sp)
+    reb = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        scene.update_device(x.data_ptr(), sp)
+        reb.append((time.perf_counter() - t0) * 1e3)
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    for _ in range(2):
+        scene.render_device(u, W, H, rgba.data_ptr(), sp)
+    ms = _events_ms(torch, stream, lambda: scene.render_device(u, W, H, rgba.data_ptr(), sp), steps)
+    info = scene.info()
+    scene.close()
+    return {"rebuild_ms": round(sorted(reb)[1], 3), "forward_ms": round(ms, 4),
+            "mrays": round(W * H / ms / 1e3, 3), "n_nodes": info["n_nodes"], "bvh_depth": info["bvh_depth"],
+            "max_stack": info["max_stack"], "device_bytes": info["device_bytes"]}
+
+
+def bench_train_c4(gt, torch, dev, sp, steps):
+    """One training step at 200k single-cluster Gaussians, one 800x800 view
+    (the forward + backward workload above): forward with checkpoint,
+    dL/dRGBA, backward, Adam with the 3DGS learning rates, GPU rebuild."""
+    import numpy as np
+    from gtrace.train import Trainer
+    raw = gt.generate_raw(seed=0, config="c4")
+    rng = np.random.Generator(np.random.PCG64(3))
+    raw_t = raw.copy()
+    raw_t[:, 0:3] += rng.normal(0.0, 0.01, (raw.shape[0], 3)).astype(np.float32)
+    W = H = 800
+    u = gt.make_uniforms(gt.camera_look_at([0.0, 0.5, 4.0], [0.0, 0.0, 0.0]), W, H, hfov_deg=50.0, T_min=1e-3,
+                         alpha_min=0.01, k=16)
+    ts = gt.Scene(raw_t, alpha_min=0.01)
+    target = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    ts.render_device(u, W, H, target.data_ptr(), sp)
+    torch.cuda.synchronize()
+    ts.close()
+    x = torch.from_numpy(raw).to(dev)
+    tr = Trainer(x, [(u, W, H, target)], alpha_min=0.01)
+    loss0 = tr.step()
+    tr.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [tr.step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    build_ms = tr.scene.info()["build_ms"]
+    tr.close()
+    return {"workload": "200000 Gaussians single cluster, one 800x800 view, k 16; step = forward with checkpoint + "
+                        "dL/dRGBA + backward + Adam (3DGS learning rates) + GPU scene rebuild",
+            "ms_per_step": round(ms, 3), "mrays": round(W * H / ms / 1e3, 3), "rebuild_ms": round(build_ms, 3),
+            "loss_first": round(loss0, 3), "loss_last": round(losses[-1], 3), "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,6 +405,7 @@ def main():
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the 3M-Gaussian 16-view forward+backward")
     ap.add_argument("--no-c4", action="store_true", help="skip the 200k-Gaussian 800x800 forward+backward")
+    ap.add_argument("--no-train", action="store_true", help="skip the device build and training-step measurements")
     ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -459,10 +524,15 @@ def main():
     if not args.no_backward:
         out["fwd_bwd"] = bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, max(2, args.steps // 2),
                                        min(args.warmup, 2), ms_step, st)
+    if not args.no_train:
+        out["device_build"] = bench_device_build(gt, torch, dev, sp, raw, u, W, H, max(2, args.steps // 2))
+        out["device_build"]["host_build_ms"] = round(t_build * 1e3, 1)
     if not args.no_batched:
         out["batched"] = bench_batched(gt, scene, torch, dev, sp, K, args.batch_cams)
     if not args.no_backward and not args.no_c4:
         out["fwd_bwd_c4"] = bench_c4(gt, torch, dev, sp, 5)
+    if not args.no_backward and not args.no_train:
+        out["train_c4"] = bench_train_c4(gt, torch, dev, sp, 10)
     if not args.no_c5:
         out["c5"] = bench_c5(gt, torch, dev, sp, 2)
 

@@ -366,28 +366,65 @@ GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
   {0, 4, 6, 0, 1, 0, 5, 1, 2, 1, 2, 9, 3, 2, 3, 8, 4, 3, 4, 7, 5, 15, 19, 5, 16, 6, 6, 15, 17, 7, \
    7, 16, 18, 8, 8, 17, 9, 19, 18, 9, 10, 14, 15, 10, 11, 10, 16, 11, 11, 12, 12, 17, 13, 12, 13, 18, 14, 13, 14, 19}
 
-/* Edge-major evaluation: each edge value is formed once and immediately
- * marks the faces whose test it fails (!(v <= margin_g): a NaN fails), so
- * only one edge value is live at a time.  No facing pre-test: a back face
- * passes "all three signed values <= margin_g" only when all three lie in
- * [0, margin_g] (its three F sum to dot(n, d) > 0 with the opposite sign
- * pattern), i.e. for faces nearly parallel to the ray, and the exact test's
- * facing rule rejects it; margin_n is kept for the interface. */
+/* Undirected edges of gt_ico_edge_tables as vertex index pairs (a, b), a < b. */
+#define GT_ICO_EDGE_VERTS                                                                         \
+  {0, 11, 5, 11, 0, 5, 1, 5, 0, 1, 1, 7, 0, 7, 7, 10, 0, 10, 10, 11, 5, 9, 1, 9, 4, 11, 4, 5, 2, 10, \
+   2, 11, 6, 7, 6, 10, 1, 8, 7, 8, 3, 9, 4, 9, 3, 4, 2, 4, 2, 3, 2, 6, 3, 6, 6, 8, 3, 8, 8, 9}
+#define GT_ICO_VERTEX_VALUES                                                                      \
+  {-GT_ICO_S, GT_ICO_T, 0.f, GT_ICO_S, GT_ICO_T, 0.f, -GT_ICO_S, -GT_ICO_T, 0.f, GT_ICO_S, -GT_ICO_T, 0.f, \
+   0.f, -GT_ICO_S, GT_ICO_T, 0.f, GT_ICO_S, GT_ICO_T, 0.f, -GT_ICO_S, -GT_ICO_T, 0.f, GT_ICO_S, -GT_ICO_T, \
+   GT_ICO_T, 0.f, -GT_ICO_S, GT_ICO_T, 0.f, GT_ICO_S, -GT_ICO_T, 0.f, -GT_ICO_S, -GT_ICO_T, 0.f, GT_ICO_S}
+
+/* Sum of the nonzero terms a[j] * c[j] in order j = 0, 1, 2 (c are
+ * literals: after unrolling the zero tests fold away). */
+GT_HD float gt_sparse_dot(gt_f3 a, float c0, float c1, float c2) {
+  float acc = 0.0f;
+  int first = 1;
+  if (c0 != 0.0f) { acc = a.x * c0; first = 0; }
+  if (c1 != 0.0f) { const float p = a.y * c1; acc = first ? p : acc + p; first = 0; }
+  if (c2 != 0.0f) { const float p = a.z * c2; acc = first ? p : acc + p; first = 0; }
+  return acc;
+}
+
+/* Face-major evaluation through vertex terms.  With w = o x d,
+ *   F(a->b) = d . (a x b) + w . b - w . a,
+ * so the 12 vertex terms W_v = w . v and the products of d with the few
+ * distinct a x b components are formed once, and an edge value costs a few
+ * adds.  A face (edges e1 e2 e3 with orientations) is a candidate iff
+ * max(+-F_e1, +-F_e2, +-F_e3) <= margin_g.  The rounding error of this
+ * evaluation is below 25 ulp * |d|_1 (|o|_1 + 1), inside margin_g with the
+ * triangle test's own error.  A NaN edge value makes fmaxf ignore it (the
+ * face stays a candidate unless all three are NaN, where the exact test
+ * rejects too).  No facing pre-test: a back face passes only when all
+ * three signed values lie in [0, margin_g] (its three F sum to
+ * dot(n, d) > 0 with the opposite sign pattern), i.e. for faces nearly
+ * parallel to the ray, and the exact test's facing rule rejects it;
+ * margin_n is kept for the interface. */
 GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, float margin_g, float margin_n) {
   const float edges[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
-  const uint8_t ef[2 * GT_NUM_EDGES] = GT_ICO_EDGE_FACES;
-  gt_f3 w = gt_cross(oo, od);
-  uint32_t bad = 0;
+  const uint8_t ev[2 * GT_NUM_EDGES] = GT_ICO_EDGE_VERTS;
+  const float vtx[36] = GT_ICO_VERTEX_VALUES;
+  const int8_t fe[3 * GT_NUM_FACES] = GT_ICO_FACE_EDGES;
+  const gt_f3 w = gt_cross(oo, od);
+  float W[12];
   (void)margin_n;
   GT_UNROLL
-  for (int e = 0; e < GT_NUM_EDGES; ++e) {
-    const float* E = edges + 6 * e;
-    float g = ((od.x * E[0] + od.y * E[1]) + od.z * E[2]) + ((w.x * E[3] + w.y * E[4]) + w.z * E[5]);
-    bad |= (uint32_t)!(g <= margin_g) << ef[2 * e];
-    bad |= (uint32_t)!(-g <= margin_g) << ef[2 * e + 1];
-    GT_SCHED_BARRIER();
+  for (int v = 0; v < 12; ++v) W[v] = gt_sparse_dot(w, vtx[3 * v], vtx[3 * v + 1], vtx[3 * v + 2]);
+  uint32_t cand = 0;
+  GT_UNROLL
+  for (int f = 0; f < GT_NUM_FACES; ++f) {
+    float val[3];
+    GT_UNROLL
+    for (int j = 0; j < 3; ++j) {
+      const int c = fe[3 * f + j], e = (c > 0 ? c : -c) - 1;
+      const float* K = edges + 6 * e;
+      const float F = (gt_sparse_dot(od, K[0], K[1], K[2]) + W[ev[2 * e + 1]]) - W[ev[2 * e]];
+      val[j] = c > 0 ? F : -F;
+    }
+    const float m = fmaxf(fmaxf(val[0], val[1]), val[2]);
+    cand |= (uint32_t)(m <= margin_g) << f;
   }
-  return ~bad & ((1u << GT_NUM_FACES) - 1u);
+  return cand;
 }
 
 /* 1 if the literal tables equal the computed ones (host check). */
@@ -406,6 +443,19 @@ static inline int gt_ico_literals_match(void) {
   }
   for (int i = 0; i < 3 * GT_NUM_FACES; ++i)
     if (fe[i] != lf[i]) return 0;
+  {
+    const uint8_t lv[2 * GT_NUM_EDGES] = GT_ICO_EDGE_VERTS;
+    const float vv[36] = GT_ICO_VERTEX_VALUES;
+    for (int i = 0; i < 36; ++i)
+      if (vv[i] != gt_ico_vertices[i]) return 0;
+    for (int e = 0; e < GT_NUM_EDGES; ++e) {
+      const gt_f3 A = gt_ico_vertex(lv[2 * e]), B = gt_ico_vertex(lv[2 * e + 1]);
+      const gt_f3 K = gt_cross(A, B), D = gt_sub(B, A);
+      if (lv[2 * e] >= lv[2 * e + 1] || K.x != le[6 * e] || K.y != le[6 * e + 1] || K.z != le[6 * e + 2] ||
+          D.x != le[6 * e + 3] || D.y != le[6 * e + 4] || D.z != le[6 * e + 5])
+        return 0;
+    }
+  }
   {
     const uint8_t ef[2 * GT_NUM_EDGES] = GT_ICO_EDGE_FACES;
     for (int f = 0; f < GT_NUM_FACES; ++f)
