@@ -106,9 +106,16 @@ int gt_scene_set_alpha_min(gt_scene* s, double alpha_min);
  * (it synchronises hip_stream).  gt_scene_set_alpha_min rebuilds on the GPU. */
 int gt_scene_create_device(const float* raw_dev, uint32_t n, double alpha_min, void* hip_stream, gt_scene** out);
 
-/* Rebuilds a gt_scene_create_device scene from updated parameters (same n,
- * same alpha_min), reusing its device memory: the step after an optimizer
- * update.  Must not overlap renders or backward calls on the scene. */
+/* gt_scene_create_raw's inputs (HOST PLY-raw floats) built on the GPU as
+ * gt_scene_create_device does: one upload, then the GPU build (synchronous).
+ * For hosts without device memory (the JavaScript host); results equal
+ * gt_scene_create_raw's. */
+int gt_scene_create_raw_gpu(const float* raw, uint32_t n, double alpha_min, gt_scene** out);
+
+/* Rebuilds a GPU-built scene (gt_scene_create_device or _raw_gpu) from
+ * updated DEVICE parameters (same n, same alpha_min), reusing its device
+ * memory: the step after an optimizer update.  Must not overlap renders or
+ * backward calls on the scene. */
 int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* hip_stream);
 
 void gt_scene_destroy(gt_scene* s);

@@ -160,3 +160,16 @@ def test_device_scene_backward_equals_host():
     scale = np.abs(out[0]).max()
     assert scale > 0
     assert np.abs(out[0] - out[1]).max() <= 1e-5 * scale
+
+
+def test_host_input_gpu_build():
+    """gt_scene_create_raw_gpu: host parameters uploaded and built on the GPU."""
+    _print("case create_raw_gpu")
+    raw = gt.generate_raw(n=20000, seed=14, config="c1")
+    h = gt.Scene(raw, alpha_min=0.1)
+    g = gt.Scene(raw, alpha_min=0.1, gpu_build=True)
+    u = uniforms([0.5, 0.2, 7.0], [0, 0, 0], 96, 80, hfov_deg=60, alpha_min=0.1, k=6, T_min=0.03)
+    _same_render(h, g, u, 96, 80, label="raw_gpu")
+    x = torch.from_numpy(raw).to("cuda:0")
+    g.update_device(x.data_ptr())  # a scene from host input can be updated from device parameters
+    _same_render(h, g, u, 96, 80, label="raw_gpu updated")

@@ -62,14 +62,17 @@ const cam = new gt.WASDCamera({{position: [0, 0, 8], target: [0, 0, 0]}});
 const a = tracer.render(cam, {{alpha_min: 0.1, hit_array_size: 6, T_min: 0.03, debugMode: 0, width: 64, height: 48}});
 const b = tracer.render(cam, {{alpha_min: 0.2, hit_array_size: 6, T_min: 0.03, debugMode: 0, width: 64, height: 48}});
 const info = tracer.info();
+const tracerGpu = new gt.GaussianTracer(raw, 4000, 0.1, {{gpuBuild: true}});
+const c = tracerGpu.render(cam, {{alpha_min: 0.1, hit_array_size: 6, T_min: 0.03, debugMode: 0, width: 64, height: 48}});
+tracerGpu.destroy();
 tracer.destroy();
 let err = '';
 try {{ tracer.render(cam, {{width: 8, height: 8}}); }} catch (e) {{ err = e.message; }}
 if (!err) throw new Error('render after destroy did not throw');
-fs.writeFileSync({json.dumps(str(out))}, Buffer.concat([a, b].map(x => Buffer.from(x.buffer))));
+fs.writeFileSync({json.dumps(str(out))}, Buffer.concat([a, b, c].map(x => Buffer.from(x.buffer))));
 console.log(JSON.stringify(info));
 """, tmp_path)
-    blob = np.fromfile(out, np.float32).reshape(2, 48, 64, 4)
+    blob = np.fromfile(out, np.float32).reshape(3, 48, 64, 4)
     raw = gt.generate_raw(n=4000, box_half=2.0, log_scale_mean=-3.0, log_scale_std=0.5, seed=0)
     tw = gt.camera_look_at([0, 0, 8], [0, 0, 0])
     for i, amin in enumerate((0.1, 0.2)):
@@ -78,3 +81,4 @@ console.log(JSON.stringify(info));
         want = s.render(u, 64, 48)["rgba"].reshape(48, 64, 4)
         np.testing.assert_array_equal(blob[i], want)
     assert not np.array_equal(blob[0], blob[1])
+    np.testing.assert_array_equal(blob[2], blob[0])  # GPU-built scene (gpuBuild) renders the same image

@@ -449,6 +449,11 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
     const uint32_t ray = py * F.width + px;
     RayState r;
     r.o = gt_f3_make(F.to_world[12], F.to_world[13], F.to_world[14]);
+    /* the origin is the same in every lane; held in VGPRs it is a legal
+     * second operand of the box and transform arithmetic against scalar
+     * node and proxy data (one scalar operand per VALU instruction), which
+     * saves a copy per operation */
+    asm volatile("" : "+v"(r.o.x), "+v"(r.o.y), "+v"(r.o.z));
     r.d = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
     {
       const float dx = fabsf(r.d.x) < 1e-20f ? copysignf(1e-20f, r.d.x) : r.d.x;

@@ -548,6 +548,37 @@ extern "C" int gt_scene_create_device(const float* raw_dev, uint32_t n, double a
   return GT_OK;
 }
 
+extern "C" int gt_scene_create_raw_gpu(const float* raw, uint32_t n, double alpha_min, gt_scene** out) {
+  if (!out) { gt_set_error("gt_scene_create_raw_gpu: out is NULL"); return GT_E_INVALID; }
+  *out = nullptr;
+  if (n && !raw) { gt_set_error("gt_scene_create_raw_gpu: raw is NULL"); return GT_E_INVALID; }
+  if (!(alpha_min > 0.0 && alpha_min < 1.0)) {
+    gt_set_error("gt_scene_create_raw_gpu: alpha_min %g outside (0, 1)", alpha_min);
+    return GT_E_INVALID;
+  }
+  if (n > GT_MAX_PROXIES) {
+    gt_set_error("gt_scene_create_raw_gpu: %u Gaussians exceed the limit %u", n, GT_MAX_PROXIES);
+    return GT_E_LIMIT;
+  }
+  std::unique_ptr<gt_scene> s(new (std::nothrow) gt_scene());
+  if (!s) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    gt_set_error("no HIP device available");
+    return GT_E_HIP;
+  }
+  s->n = n;
+  s->device_built = true;
+  s->alpha_min = alpha_min;
+  const size_t bytes = (size_t)n * GT_RAW_FLOATS * sizeof(float);
+  int rc = gt_devbuf_reserve(s->raw_dev, bytes);
+  if (rc != GT_OK) return rc;
+  if (bytes) HIP_TRY(hipMemcpy(s->raw_dev.p, raw, bytes, hipMemcpyHostToDevice));
+  rc = gt_device_build(s.get(), nullptr);
+  if (rc != GT_OK) return rc;
+  *out = s.release();
+  return GT_OK;
+}
+
 extern "C" int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* stream) {
   if (!s) { gt_set_error("gt_scene_update_device: scene is NULL"); return GT_E_INVALID; }
   if (!s->device_built) {

@@ -107,6 +107,7 @@ _lib.gt_scene_create_packed.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P
 _lib.gt_scene_set_alpha_min.argtypes = [_P, C.c_double]
 _lib.gt_scene_create_device.argtypes = [_P, C.c_uint32, C.c_double, _P, C.POINTER(_P)]
 _lib.gt_scene_update_device.argtypes = [_P, _P, _P]
+_lib.gt_scene_create_raw_gpu.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
 _lib.gt_scene_destroy.argtypes = [_P]
 _lib.gt_scene_destroy.restype = None
 _lib.gt_scene_get_info.argtypes = [_P, C.POINTER(SceneInfo)]
@@ -135,7 +136,7 @@ _lib.gt_free.restype = None
 
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
-    "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_update_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
+    "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_create_raw_gpu", "gt_scene_update_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
     "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
@@ -261,12 +262,14 @@ def host_prepare(raw, alpha_min):
 class Scene:
     """Acceleration container over the Gaussians (gaussianSceneToBvh, src/main.ts:30-111).
 
-    raw / packed: host arrays, built on the host (binned SAH).  raw_dev: a
+    raw / packed: host arrays, built on the host (binned SAH), or for raw
+    with gpu_build=True uploaded and built on the GPU.  raw_dev: a
     device pointer to n x 59 PLY-raw floats (e.g. a torch tensor's
     data_ptr()), built on the GPU (gt_scene_create_device); update_device()
     rebuilds it after the parameters change."""
 
-    def __init__(self, raw=None, alpha_min=0.2, packed=None, raw_dev=None, n=None, stream_ptr=None):
+    def __init__(self, raw=None, alpha_min=0.2, packed=None, raw_dev=None, n=None, stream_ptr=None,
+                 gpu_build=False):
         self._h = C.c_void_p()
         if raw_dev is not None:
             if n is None:
@@ -275,7 +278,8 @@ class Scene:
         elif raw is not None:
             raw = np.ascontiguousarray(raw, dtype=np.float32)
             assert raw.ndim == 2 and raw.shape[1] == GT_RAW_FLOATS
-            _check(_lib.gt_scene_create_raw(_ptr(raw), raw.shape[0], float(alpha_min), C.byref(self._h)))
+            create = _lib.gt_scene_create_raw_gpu if gpu_build else _lib.gt_scene_create_raw
+            _check(create(_ptr(raw), raw.shape[0], float(alpha_min), C.byref(self._h)))
         elif packed is not None:
             packed = np.ascontiguousarray(packed, dtype=np.float32).reshape(-1, 80)
             _check(_lib.gt_scene_create_packed(_ptr(packed), packed.shape[0], float(alpha_min), C.byref(self._h)))

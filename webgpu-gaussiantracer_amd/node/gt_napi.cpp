@@ -210,8 +210,17 @@ napi_value js_camera_look_at(napi_env env, napi_callback_info info) {
 }
 
 napi_value js_scene_create(napi_env env, napi_callback_info info) {
-  napi_value argv[3];
-  if (!get_args(env, info, 3, argv)) return nullptr;
+  napi_value argv[4];
+  size_t argc = 4;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < 3) {
+    napi_throw_type_error(env, nullptr, "sceneCreate(raw: Float32Array, n: number, alphaMin: number, gpuBuild?: boolean)");
+    return nullptr;
+  }
+  bool gpu = false;
+  if (argc >= 4) {
+    napi_valuetype t;
+    if (napi_typeof(env, argv[3], &t) == napi_ok && t == napi_boolean) napi_get_value_bool(env, argv[3], &gpu);
+  }
   float* raw = nullptr;
   size_t len = 0;
   uint32_t n = 0;
@@ -226,7 +235,8 @@ napi_value js_scene_create(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   gt_scene* s = nullptr;
-  const int rc = gt_scene_create_raw(raw, n, amin, &s);
+  /* gpuBuild: upload once and build on the GPU (gt_scene_create_raw_gpu) */
+  const int rc = gpu ? gt_scene_create_raw_gpu(raw, n, amin, &s) : gt_scene_create_raw(raw, n, amin, &s);
   if (rc != GT_OK) return throw_gt(env, rc);
   Handle* h = new Handle{s};
   napi_value ext;

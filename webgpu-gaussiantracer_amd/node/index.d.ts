@@ -57,7 +57,7 @@ export declare class WASDCamera {
 }
 
 export declare class GaussianTracer {
-  constructor(raw: Float32Array, n: number, alphaMin?: number);
+  constructor(raw: Float32Array, n: number, alphaMin?: number, opts?: { gpuBuild?: boolean });
   render(camera: WASDCamera | Float32Array, opts: RenderOptions): Float32Array;  // width*height*4
   info(): SceneInfo;
   destroy(): void;

@@ -56,11 +56,13 @@ function generateScene(params) {
 
 class GaussianTracer {
   /* raw: Float32Array of n*59 PLY-raw floats; alphaMin: the proxy filter /
-   * size threshold of gaussianSceneToBvh (src/main.ts:30-111) */
-  constructor(raw, n, alphaMin = DEFAULTS.alpha_min) {
+   * size threshold of gaussianSceneToBvh (src/main.ts:30-111);
+   * opts.gpuBuild: build the proxies and the BVH on the GPU (same results,
+   * milliseconds instead of seconds for a million Gaussians) */
+  constructor(raw, n, alphaMin = DEFAULTS.alpha_min, opts = {}) {
     this.n = n;
     this.alphaMin = alphaMin;
-    this.handle = addon.sceneCreate(raw, n, alphaMin);
+    this.handle = addon.sceneCreate(raw, n, alphaMin, !!opts.gpuBuild);
   }
 
   /* render(camera, {alpha_min, hit_array_size, T_min, debugMode, ...}):
