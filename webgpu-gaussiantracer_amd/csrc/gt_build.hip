@@ -287,18 +287,28 @@ __device__ __forceinline__ void child_data(const BNode* bn, const float4* lbox, 
   }
 }
 
-/* one thread per leaf climbs while it is the second child to arrive */
+/* One thread per leaf climbs while it is the second child to arrive.  A
+ * node whose leaves all belong to this block's threads is completed by
+ * threads of this block only, so its arrival handshake needs workgroup
+ * scope; only nodes whose range crosses a block boundary (a few percent)
+ * pay the device-scope fences (an L2 write-back across XCDs). */
 __global__ __launch_bounds__(BUILD_BLOCK) void k_refit(uint32_t np, BNode* __restrict__ bn,
                                                        const float4* __restrict__ lbox,
                                                        const uint32_t* __restrict__ lparent, uint32_t* visit) {
   const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
   if (p >= np) return;
+  const uint32_t blo = blockIdx.x * BUILD_BLOCK, bhi = min(blo + BUILD_BLOCK, np); /* this block's leaves */
   uint32_t node = lparent[p];
   while (node != BIN_NONE) {
-    __threadfence(); /* release this thread's writes below node */
-    if (atomicAdd(visit + node, 1u) == 0u) return;
-    __threadfence(); /* acquire the sibling's */
     BNode* N = bn + node;
+    const bool local = N->first >= blo && N->first + N->count <= bhi;
+    if (local) {
+      if (__hip_atomic_fetch_add(visit + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+    } else {
+      __threadfence(); /* release this thread's writes below node */
+      if (atomicAdd(visit + node, 1u) == 0u) return;
+      __threadfence(); /* acquire the sibling's */
+    }
     float llo[3], lhi[3], rlo[3], rhi[3];
     double cl, cr;
     child_data(bn, lbox, N->left, llo, lhi, &cl);
