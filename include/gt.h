@@ -206,6 +206,14 @@ typedef struct gt_frame {
    * n_comp without comp_ids only counts, to size a checkpoint. */
   uint32_t* comp_ids;
   uint32_t* n_comp;
+  /* Nonzero: the reference's antialiasing path (compiled out there,
+   * GS_shader.ts:54, 282-307): each pixel's ray goes through a jittered
+   * offset drawn from (y*W+x, u.accumulated_frames), and rgba, which holds
+   * the average of the previous frames on input, receives
+   * frames == 0 ? new : mix(old, new, 1 / (frames + 1)).  Single-camera
+   * renders only (not batches), and not with the backward checkpoint. */
+  uint32_t antialiasing;
+  uint32_t _pad;
 } gt_frame;
 
 /* traceRays(W, H) of the reference with the uniform block u.  Unlike the

@@ -473,15 +473,54 @@ static inline int gt_ico_literals_match(void) {
  *   X = -col0, Y = col1, Z = -col2
  *   ndc = (id + 0.5) / size * 2 - 1
  *   dir = normalize(ndc.x*a.x*X + ndc.y*a.y*Y + Z)                       */
-GT_HD gt_f3 gt_ray_dir(const float* to_world, float aspect_x, float aspect_y,
-                       uint32_t px, uint32_t py, uint32_t width, uint32_t height) {
+GT_HD gt_f3 gt_ray_dir_off(const float* to_world, float aspect_x, float aspect_y, uint32_t px, uint32_t py,
+                           uint32_t width, uint32_t height, float off_x, float off_y) {
   gt_f3 X = gt_f3_make(-to_world[0], -to_world[1], -to_world[2]);
   gt_f3 Y = gt_f3_make(to_world[4], to_world[5], to_world[6]);
   gt_f3 Z = gt_f3_make(-to_world[8], -to_world[9], -to_world[10]);
-  float nx = ((float)px + 0.5f) / (float)width * 2.0f - 1.0f;
-  float ny = ((float)py + 0.5f) / (float)height * 2.0f - 1.0f;
+  float nx = ((float)px + off_x) / (float)width * 2.0f - 1.0f;
+  float ny = ((float)py + off_y) / (float)height * 2.0f - 1.0f;
   gt_f3 v = gt_add(gt_add(gt_scale(nx * aspect_x, X), gt_scale(ny * aspect_y, Y)), Z);
   return gt_normalize(v);
+}
+GT_HD gt_f3 gt_ray_dir(const float* to_world, float aspect_x, float aspect_y,
+                       uint32_t px, uint32_t py, uint32_t width, uint32_t height) {
+  return gt_ray_dir_off(to_world, aspect_x, aspect_y, px, py, width, height, 0.5f, 0.5f);
+}
+
+/* Antialiasing jitter of the reference's ray generation, compiled out there
+ * (antialiasing = false, GS_shader.ts:54) and opt-in here.  The pixel seed
+ * is getNewSeed(y*W + x, accumulatedFrames, 8) (GS_shader.ts:8-22): rounds
+ * of the Tiny Encryption Algorithm (Wheeler & Needham 1994) with the
+ * shader's four 32-bit key words, returning the first word.  rng
+ * (GS_shader.ts:24-28) is the LCG seed = 1664525 seed + 1013904223 with the
+ * low 24 bits scaled to [0, 1) (exact in FP32).  All integer arithmetic:
+ * identical on host and device. */
+GT_HD uint32_t gt_aa_seed(uint32_t v0, uint32_t v1, uint32_t rounds) {
+  uint32_t sum = 0;
+  for (uint32_t i = 0; i < rounds; ++i) {
+    sum += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+GT_HD float gt_aa_rng(uint32_t* seed) {
+  *seed = 1664525u * *seed + 1013904223u;
+  return (float)(*seed & 0x00ffffffu) / (float)0x01000000u;
+}
+/* pixel offset of pixel (px, py) in frame `frames` (GS_shader.ts:282-283) */
+GT_HD void gt_aa_offset(uint32_t px, uint32_t py, uint32_t width, uint32_t frames, float* off_x, float* off_y) {
+  uint32_t seed = gt_aa_seed(width * py + px, frames, 8u);
+  *off_x = gt_aa_rng(&seed);
+  *off_y = gt_aa_rng(&seed);
+}
+/* running average of the accumulated frames (GS_shader.ts:303-307):
+ * frames == 0 ? c : mix(prev, c, 1 / (frames + 1)), mix(x, y, a) = x (1 - a) + y a */
+GT_HD float gt_aa_mix(float prev, float c, uint32_t frames) {
+  if (frames == 0) return c;
+  const float a = 1.0f / ((float)frames + 1.0f);
+  return prev * (1.0f - a) + c * a;
 }
 
 /* Object-space ray (trace.glsl:102-105): w2o is the 4x3 column-major

@@ -119,15 +119,24 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
 double gto_exp64(double x) { return gt_exp64(x); }
 double gto_log64(double x) { return gt_log64(x); }
 
+/* pixel offset (GS_shader.ts:281-283): 0.5, or the getNewSeed/rng jitter with antialiasing */
+static void pixel_offset(const gto_params* p, uint32_t px, uint32_t py, float* ox, float* oy) {
+  *ox = 0.5f;
+  *oy = 0.5f;
+  if (p->antialias) gt_aa_offset(px, py, p->width, p->accumulated_frames, ox, oy);
+}
+
 void gto_ray_dir(const gto_params* p, uint32_t px, uint32_t py, float* d3) {
   gt_f3 d;
+  float ox, oy;
+  pixel_offset(p, px, py, &ox, &oy);
   if (g_mutation == 3) {
     float tw[16];
     memcpy(tw, p->to_world, sizeof(tw));
     tw[0] = -tw[0]; tw[1] = -tw[1]; tw[2] = -tw[2];
-    d = gt_ray_dir(tw, p->aspect[0], p->aspect[1], px, py, p->width, p->height);
+    d = gt_ray_dir_off(tw, p->aspect[0], p->aspect[1], px, py, p->width, p->height, ox, oy);
   } else {
-    d = gt_ray_dir(p->to_world, p->aspect[0], p->aspect[1], px, py, p->width, p->height);
+    d = gt_ray_dir_off(p->to_world, p->aspect[0], p->aspect[1], px, py, p->width, p->height, ox, oy);
   }
   d3[0] = d.x; d3[1] = d.y; d3[2] = d.z;
 }
@@ -518,6 +527,8 @@ static void render_ray(render_job* J, uint32_t li, uint64_t* st) {
   }
   if (J->rgba) {
     float* o = J->rgba + 4 * (size_t)li;
+    if (p->antialias) /* GS_shader.ts:303-307: running average over accumulated frames */
+      for (int c = 0; c < 3; ++c) L[c] = gt_aa_mix(o[c], L[c], p->accumulated_frames);
     o[0] = L[0]; o[1] = L[1]; o[2] = L[2]; o[3] = 1.0f;
   }
   if (J->T_out) J->T_out[li] = T;
@@ -648,8 +659,10 @@ void gto_reeval_f64(const gto_scene* s, const gto_params* p, const uint32_t* ray
   for (uint32_t li = 0; li < n; ++li) {
     uint32_t ray = ray_list ? ray_list[li] : li;
     uint32_t px = ray % p->width, py = ray / p->width;
-    double nx = ((double)px + 0.5) / p->width * 2.0 - 1.0;
-    double ny = ((double)py + 0.5) / p->height * 2.0 - 1.0;
+    float ox, oy;
+    pixel_offset(p, px, py, &ox, &oy);
+    double nx = ((double)px + ox) / p->width * 2.0 - 1.0;
+    double ny = ((double)py + oy) / p->height * 2.0 - 1.0;
     double d[3];
     for (int j = 0; j < 3; ++j) d[j] = nx * p->aspect[0] * X[j] + ny * p->aspect[1] * Y[j] + Z[j];
     double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);

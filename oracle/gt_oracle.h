@@ -47,6 +47,7 @@ typedef struct gto_params {
   float t_min, t_max, T_min, alpha_min;
   uint32_t k, sh_degree_max, accumulated_frames, early_stop, debug_mode;
   uint32_t width, height;
+  uint32_t antialias; /* GS_shader.ts's antialiasing path: jittered rays, running average (rgba in/out) */
 } gto_params;
 
 /* stats[]: 0 rays, 1 rounds, 2 H_eval, 3 H_comp, 4 proxy candidates tested,
@@ -62,6 +63,8 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
                            uint32_t* proxy_ids, float* o2w, float* w2o);
 
 /* Render.  ray_list (optional) selects rays by linear index y*W+x; outputs
+ * With p->antialias and accumulated_frames > 0, rgba holds the running
+ * average on input and receives the new average (GS_shader.ts:303-307).
  * are indexed by position in the list.  hit_ids: per ray max_rounds*k ids
  * (sentinel-filled), accept: per ray max_rounds u32 masks (bit i = entry i
  * composited), n_rounds: rounds run.  Any output may be NULL.
@@ -76,7 +79,8 @@ int gto_render(const gto_scene* s, const gto_params* p, int use_bvh, int n_threa
 void* gto_bvh_build(const gto_scene* s);
 void gto_bvh_free(void* b);
 
-/* FP64 re-evaluation of given hit lists and accept masks: pixels in double. */
+/* FP64 re-evaluation of given hit lists and accept masks: pixels in double
+ * (of the frame itself: the antialiasing average is not applied). */
 void gto_reeval_f64(const gto_scene* s, const gto_params* p, const uint32_t* ray_list, uint32_t n_list,
                     const uint32_t* n_rounds, uint32_t max_rounds, const uint32_t* hit_ids,
                     const uint32_t* accept, double* rgb_out, double* T_out);

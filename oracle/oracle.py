@@ -32,7 +32,7 @@ class Params(C.Structure):
                 ("t_min", C.c_float), ("t_max", C.c_float), ("T_min", C.c_float), ("alpha_min", C.c_float),
                 ("k", C.c_uint32), ("sh_degree_max", C.c_uint32), ("accumulated_frames", C.c_uint32),
                 ("early_stop", C.c_uint32), ("debug_mode", C.c_uint32),
-                ("width", C.c_uint32), ("height", C.c_uint32)]
+                ("width", C.c_uint32), ("height", C.c_uint32), ("antialias", C.c_uint32)]
 
 
 _P = C.c_void_p
@@ -119,8 +119,9 @@ class OracleScene:
         return self._s
 
 
-def params_from_uniforms(u, width, height):
+def params_from_uniforms(u, width, height, antialias=False):
     p = Params()
+    p.antialias = 1 if antialias else 0
     for i in range(16):
         p.to_world[i] = u.to_world[i]
     p.aspect[0], p.aspect[1] = u.camera_aspect[0], u.camera_aspect[1]
@@ -131,13 +132,14 @@ def params_from_uniforms(u, width, height):
     return p
 
 
-def render(scene, params, max_rounds=0, use_bvh=True, threads=None, ray_list=None):
-    """Returns dict(rgba (R,4), T, n_rounds, hit_ids (R,MR,k), accept (R,MR), stats)."""
+def render(scene, params, max_rounds=0, use_bvh=True, threads=None, ray_list=None, rgba_prev=None):
+    """Returns dict(rgba (R,4), T, n_rounds, hit_ids (R,MR,k), accept (R,MR), stats).
+    rgba_prev: the antialiasing running average the new frame is averaged into."""
     if threads is None:
         threads = os.cpu_count() or 1
     R = params.width * params.height if ray_list is None else len(ray_list)
     rl = None if ray_list is None else np.ascontiguousarray(ray_list, np.uint32)
-    rgba = np.zeros((R, 4), np.float32)
+    rgba = np.zeros((R, 4), np.float32) if rgba_prev is None else np.array(rgba_prev, np.float32).reshape(R, 4)
     T = np.zeros(R, np.float32)
     rounds = np.zeros(R, np.uint32)
     hits = acc = None

@@ -82,3 +82,32 @@ console.log(JSON.stringify(info));
         np.testing.assert_array_equal(blob[i], want)
     assert not np.array_equal(blob[0], blob[1])
     np.testing.assert_array_equal(blob[2], blob[0])  # GPU-built scene (gpuBuild) renders the same image
+
+
+@pytest.mark.gpu
+def test_antialias_through_addon(tmp_path):
+    print("case node addon antialiasing", flush=True)
+    out = tmp_path / "aa.bin"
+    _node(f"""
+const raw = gt.generateScene({{n: 4000, boxHalf: 2, logScaleMean: -3, logScaleStd: 0.5, seed: 0}});
+const tracer = new gt.GaussianTracer(raw, 4000, 0.1);
+const cam = new gt.WASDCamera({{position: [0, 0, 8], target: [0, 0, 0]}});
+const o = {{alpha_min: 0.1, hit_array_size: 6, T_min: 0.03, width: 40, height: 30, antialiasing: true}};
+const f0 = tracer.render(cam, Object.assign({{}}, o, {{accumulatedFrames: 0}}));
+const f1 = tracer.render(cam, Object.assign({{}}, o, {{accumulatedFrames: 1, prev: f0}}));
+let err = '';
+try {{ tracer.render(cam, Object.assign({{}}, o, {{accumulatedFrames: 2}})); }} catch (e) {{ err = e.message; }}
+if (!err.includes('prev')) throw new Error('missing prev not reported: ' + err);
+tracer.destroy();
+fs.writeFileSync({json.dumps(str(out))}, Buffer.concat([f0, f1].map(x => Buffer.from(x.buffer))));
+""", tmp_path)
+    blob = np.fromfile(out, np.float32).reshape(2, 30, 40, 4)
+    raw = gt.generate_raw(n=4000, box_half=2.0, log_scale_mean=-3.0, log_scale_std=0.5, seed=0)
+    tw = gt.camera_look_at([0, 0, 8], [0, 0, 0])
+    s = gt.Scene(raw, alpha_min=0.1)
+    prev = None
+    for frames in range(2):
+        u = gt.make_uniforms(tw, 40, 30, hfov_deg=60, alpha_min=0.1, k=6, T_min=0.03, accumulated_frames=frames)
+        want = s.render(u, 40, 30, antialiasing=True, rgba_prev=prev)["rgba"]
+        np.testing.assert_array_equal(blob[frames], want)
+        prev = want

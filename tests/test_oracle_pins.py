@@ -138,6 +138,50 @@ def check_ray_dir():
     assert np.abs((d00 + dWH) / 2 - (d00 + dWH) @ Z / 2 * Z / (Z @ Z)).max() < 1e-6
 
 
+def _reference_pixel_offset(px, py, W, frames):
+    """GS_shader.ts:8-28, 281-283 restated in Python integers: seed =
+    getNewSeed(W*py + px, frames, 8) (TEA rounds, first word), then two
+    rng draws (LCG, low 24 bits / 2^24)."""
+    m = 0xFFFFFFFF
+    v0, v1, s0 = (W * py + px) & m, frames & m, 0
+    for _ in range(8):
+        s0 = (s0 + 0x9E3779B9) & m
+        v0 = (v0 + ((((v1 << 4) & m) + 0xA341316C) & m ^ ((v1 + s0) & m) ^ (((v1 >> 5) + 0xC8013EA4) & m))) & m
+        v1 = (v1 + ((((v0 << 4) & m) + 0xAD90777D) & m ^ ((v0 + s0) & m) ^ (((v0 >> 5) + 0x7E95761E) & m))) & m
+    seed, offs = v0, []
+    for _ in range(2):
+        seed = (1664525 * seed + 1013904223) & m
+        offs.append((seed & 0x00FFFFFF) / float(0x01000000))
+    return offs
+
+
+def check_aa_jitter():
+    """Antialiasing pixel offsets (compiled out in the reference): the oracle's
+    jittered ray equals the camera ray through (px + ox, py + oy) with the
+    offsets of an independent restatement of getNewSeed/rng; offsets change
+    with the frame index and are uniform on [0, 1)."""
+    pos, tgt = [3.0, 2.0, 7.0], [0.2, -0.1, 0.3]
+    tw = gt.camera_look_at(pos, tgt)
+    W, H = 33, 17
+    M = tw.astype(np.float64).reshape(4, 4)
+    X, Y, Z = -M[0, :3], M[1, :3], -M[2, :3]
+    ax = math.tan(0.5 * 50 * math.pi / 180)
+    ay = float(F32(ax)) * H / W
+    for frames in (0, 1, 7, 123456):
+        u = gt.make_uniforms(tw, W, H, hfov_deg=50, accumulated_frames=frames)
+        p = orc.params_from_uniforms(u, W, H, antialias=True)
+        for px, py in [(0, 0), (32, 16), (5, 11), (16, 8)]:
+            ox, oy = _reference_pixel_offset(px, py, W, frames)
+            ndc = np.array([(px + ox) / W * 2 - 1, (py + oy) / H * 2 - 1])
+            v = ndc[0] * ax * X + ndc[1] * ay * Y + Z
+            v /= np.linalg.norm(v)
+            d = orc.ray_dir(p, px, py)
+            assert np.abs(d - v).max() < 2e-6, (frames, px, py, d, v)
+    offs = np.array([_reference_pixel_offset(i % 97, i // 97, 97, 3) for i in range(97 * 40)])
+    assert offs.min() >= 0.0 and offs.max() < 1.0 and abs(offs.mean() - 0.5) < 0.02
+    assert _reference_pixel_offset(5, 3, 33, 1) != _reference_pixel_offset(5, 3, 33, 2)
+
+
 def check_proxy_tangent():
     """Every proxy face is tangent to the Mahalanobis sphere of radius
     s = sqrt(2 ln(opacity / alpha_min)) around the mean (main.ts:84-87),
@@ -405,7 +449,7 @@ def check_invariants_and_reeval():
 
 
 PINS = {
-    "exp64": check_exp64, "log64": check_log64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir,
+    "exp64": check_exp64, "log64": check_log64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir, "aa_jitter": check_aa_jitter,
     "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
     "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
     "composite": check_composite_closed_form, "relaunch": check_relaunch_window,

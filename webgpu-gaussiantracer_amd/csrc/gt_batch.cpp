@@ -121,6 +121,10 @@ extern "C" int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, co
       gt_set_error("gt_batch_create: camera %u: batches render debugMode 0 only", i);
       rc = GT_E_INVALID;
     }
+    if (rc == GT_OK && outs[i].antialiasing) {
+      gt_set_error("gt_batch_create: camera %u: antialiasing is a single-camera gt_render option", i);
+      rc = GT_E_INVALID;
+    }
     if (rc == GT_OK && (!outs[i].rgba || (outs[i].hit_ids != nullptr) != dump ||
                         (outs[i].accept != nullptr) != dump || (dump && outs[i].max_rounds == 0))) {
       gt_set_error("gt_batch_create: camera %u: rgba required; hit_ids/accept/max_rounds given for all or none", i);

@@ -454,7 +454,9 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
      * node and proxy data (one scalar operand per VALU instruction), which
      * saves a copy per operation */
     asm volatile("" : "+v"(r.o.x), "+v"(r.o.y), "+v"(r.o.z));
-    r.d = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
+    float ox = 0.5f, oy = 0.5f; /* pixel offset, jittered with antialiasing (GS_shader.ts:282-284) */
+    if (F.antialias) gt_aa_offset(px, py, F.width, F.accumulated_frames, &ox, &oy);
+    r.d = gt_ray_dir_off(F.to_world, F.ax, F.ay, px, py, F.width, F.height, ox, oy);
     {
       const float dx = fabsf(r.d.x) < 1e-20f ? copysignf(1e-20f, r.d.x) : r.d.x;
       const float dy = fabsf(r.d.y) < 1e-20f ? copysignf(1e-20f, r.d.y) : r.d.y;
@@ -932,6 +934,29 @@ hipError_t ensure_faces() {
 static void tuning_defaults(const gt_tuning* t, uint32_t* tile_w, uint32_t* lds_depth) {
   *tile_w = (t && t->tile_w) ? t->tile_w : 8u;
   *lds_depth = (t && t->stack_lds) ? t->stack_lds : (uint32_t)GT_LDS_STACK;
+}
+
+/* Antialiasing running average (GS_shader.ts:303-307): avg = mix(avg, img,
+ * 1 / (frames + 1)) per RGB channel, A = 1.  A separate pass so that the
+ * render kernel's register allocation is that of the plain frame. */
+__global__ __launch_bounds__(256) void k_aa_mix(const float4* __restrict__ img, float4* __restrict__ avg, uint32_t n,
+                                                uint32_t frames) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 c = img[i], a = avg[i];
+  avg[i] = make_float4(gt_aa_mix(a.x, c.x, frames), gt_aa_mix(a.y, c.y, frames), gt_aa_mix(a.z, c.z, frames), 1.0f);
+}
+
+extern "C" int gt_launch_aa_mix(const float* img, float* avg, uint32_t n_pixels, uint32_t frames, void* stream) {
+  if (n_pixels == 0) return GT_OK;
+  k_aa_mix<<<(n_pixels + 255u) / 256u, 256, 0, (hipStream_t)stream>>>(reinterpret_cast<const float4*>(img),
+                                                                       reinterpret_cast<float4*>(avg), n_pixels, frames);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    gt_set_error("k_aa_mix launch failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  return GT_OK;
 }
 
 extern "C" int gt_launch_colors(const gt_dev_scene* S, const gt_dev_frame* F, float* out, void* stream) {

@@ -62,6 +62,8 @@ typedef struct gt_dev_frame {
   uint32_t k, sh_degree, early_stop, debug_mode;
   uint32_t width, height;
   float slack_abs; /* box-test slack, |camera origin| dependent */
+  uint32_t antialias;          /* nonzero: jittered rays, running average into rgba */
+  uint32_t accumulated_frames; /* frames already averaged into rgba (AA) */
   uint32_t max_rounds;
   uint32_t tiles_x;   /* 8x8 pixel tiles per row */
   uint32_t tile_base; /* first global tile index of this camera in a batch */
@@ -99,6 +101,8 @@ int gt_launch_persistent(const gt_dev_scene* s, const gt_dev_frame* cam0, const 
 /* out[i] (float4, N entries) = FP32 radiance of Gaussian i seen from f's
  * camera origin, bit-identical to the colour the render kernels composite. */
 int gt_launch_colors(const gt_dev_scene* s, const gt_dev_frame* f, float* out, void* stream);
+/* avg[i] = gt_aa_mix(avg[i], img[i], frames) per RGB channel, A = 1 (float4 pixels) */
+int gt_launch_aa_mix(const float* img, float* avg, uint32_t n_pixels, uint32_t frames, void* stream);
 void gt_set_error(const char* fmt, ...);
 #ifdef __cplusplus
 }

@@ -659,6 +659,7 @@ void gt_fill_dev_frame(const gt_uniforms* u, uint32_t width, uint32_t height, gt
   f->height = height;
   const float om = std::max(std::fabs(u->to_world[12]), std::max(std::fabs(u->to_world[13]), std::fabs(u->to_world[14])));
   f->slack_abs = 2e-5f * (om + 1.0f);
+  f->accumulated_frames = u->accumulated_frames;
   f->tiles_x = (width + 7u) / 8u;
   f->tile_base = 0;
 }
@@ -677,8 +678,8 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
 
 static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   const uint32_t tiles = gt_frame_tiles(&s->tune, f, 0);
-  /* the checkpoint for the backward pass is written by the tile kernel */
-  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids || f->n_comp)
+  /* the checkpoint for the backward pass and antialiasing are the tile kernel's */
+  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids || f->n_comp || f->antialias)
     return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream, 0);
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
                               &s->tune, stream);
@@ -747,8 +748,13 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     gt_set_error("gt_render: the checkpoint is written in debugMode 0 only");
     return GT_E_INVALID;
   }
+  if (out->antialiasing && (out->comp_ids || out->n_comp)) {
+    gt_set_error("gt_render: antialiasing renders do not write the backward checkpoint");
+    return GT_E_INVALID;
+  }
   gt_dev_frame f;
   gt_fill_dev_frame(u, width, height, &f);
+  f.antialias = out->antialiasing ? 1u : 0u;
   const size_t R = (size_t)width * height;
   const size_t MR = out->max_rounds;
   if (buf_flags == GT_BUF_DEVICE) {
@@ -762,6 +768,12 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     f.comp_ids = out->comp_ids;
     f.n_comp = out->n_comp;
     f.comp_cap = out->comp_cap;
+    if (f.antialias && f.accumulated_frames) { /* render into scratch, then average into rgba */
+      if ((rc = gt_devbuf_reserve(s->aa, R * 16)) != GT_OK) return rc;
+      f.rgba = (float*)s->aa.p;
+      if ((rc = launch_frame(s, &f, stream)) != GT_OK) return rc;
+      return gt_launch_aa_mix(f.rgba, out->rgba, (uint32_t)R, f.accumulated_frames, stream);
+    }
     return launch_frame(s, &f, stream);
   }
   // host buffers: device temporaries, synchronous
@@ -812,8 +824,18 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
   }
   f.max_rounds = (uint32_t)MR;
   if ((rc = launch_frame(s, &f, nullptr)) != GT_OK) return rc;
-  HIP_TRY(hipDeviceSynchronize());
-  HIP_TRY(hipMemcpy(out->rgba, f.rgba, R * 16, hipMemcpyDeviceToHost));
+  if (f.antialias && f.accumulated_frames) { /* average the new frame into the caller's image */
+    Tmp t_avg;
+    if ((rc = alloc(t_avg, R * 16)) != GT_OK) return rc;
+    HIP_TRY(hipMemcpy(t_avg.p, out->rgba, R * 16, hipMemcpyHostToDevice));
+    if ((rc = gt_launch_aa_mix(f.rgba, (float*)t_avg.p, (uint32_t)R, f.accumulated_frames, nullptr)) != GT_OK)
+      return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out->rgba, t_avg.p, R * 16, hipMemcpyDeviceToHost));
+  } else {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out->rgba, f.rgba, R * 16, hipMemcpyDeviceToHost));
+  }
   if (out->hit_ids) HIP_TRY(hipMemcpy(out->hit_ids, f.hit_ids, R * MR * u->k * 4, hipMemcpyDeviceToHost));
   if (out->accept) HIP_TRY(hipMemcpy(out->accept, f.accept, R * MR * 4, hipMemcpyDeviceToHost));
   if (out->n_rounds) HIP_TRY(hipMemcpy(out->n_rounds, f.n_rounds, R * 4, hipMemcpyDeviceToHost));

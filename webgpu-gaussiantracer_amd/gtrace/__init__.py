@@ -60,6 +60,7 @@ class Frame(C.Structure):
         ("rgba", C.c_void_p), ("hit_ids", C.c_void_p), ("accept", C.c_void_p),
         ("n_rounds", C.c_void_p), ("T", C.c_void_p), ("stats", C.c_void_p),
         ("max_rounds", C.c_uint32), ("comp_cap", C.c_uint32), ("comp_ids", C.c_void_p), ("n_comp", C.c_void_p),
+        ("antialiasing", C.c_uint32), ("_pad", C.c_uint32),
     ]
 
 
@@ -169,12 +170,13 @@ def camera_look_at(pos, target):
 
 
 def make_uniforms(to_world, width, height, hfov_deg=60.0, t_min=1e-3, t_max=1e5, T_min=0.03,
-                  alpha_min=0.2, k=6, sh_degree_max=3, early_stop=0, debug_mode=0):
+                  alpha_min=0.2, k=6, sh_degree_max=3, early_stop=0, debug_mode=0, accumulated_frames=0):
     """upadateUniformData (src/main.ts:520-534) with the reference's defaults (448-464)."""
     u = Uniforms()
     tw = (C.c_float * 16)(*[float(x) for x in np.asarray(to_world, dtype=np.float32).ravel()])
     _lib.gt_uniforms_init(C.byref(u), tw, float(hfov_deg), width, height, t_min, t_max, T_min, alpha_min,
                           k, sh_degree_max, early_stop, debug_mode)
+    u.accumulated_frames = accumulated_frames
     return u
 
 
@@ -333,16 +335,22 @@ class Scene:
         _check(_lib.gt_scene_get_host_data(self._h, _ptr(packed), _ptr(ids), _ptr(o2w), _ptr(w2o)))
         return dict(packed=packed, proxy_ids=ids, o2w=o2w, w2o=w2o)
 
-    def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False, checkpoint_cap=0):
+    def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False, checkpoint_cap=0,
+               antialiasing=False, rgba_prev=None):
         """traceRays(W, H) with host buffers: returns rgba (H, W, 4) and, on request,
         hit lists (H*W, max_rounds, k), accept masks, round counts, T, stats and
         the backward checkpoint (composited ids per ray, capacity checkpoint_cap;
-        checkpoint_cap=-1 returns only the per-ray counts)."""
+        checkpoint_cap=-1 returns only the per-ray counts).  antialiasing: the
+        reference's jittered-ray path; with u.accumulated_frames > 0 the frame
+        is averaged into rgba_prev (the previous average)."""
         R = width * height
         rgba = np.zeros((height, width, 4), np.float32)
+        if rgba_prev is not None:
+            rgba[...] = np.asarray(rgba_prev, np.float32).reshape(height, width, 4)
         out = {"rgba": rgba}
         f = Frame()
         f.rgba = _ptr(rgba)
+        f.antialiasing = 1 if antialiasing else 0
         if max_rounds:
             hits = np.zeros((R, max_rounds, u.k), np.uint32)
             acc = np.zeros((R, max_rounds), np.uint32)
@@ -372,11 +380,13 @@ class Scene:
         return out
 
     def render_device(self, u, width, height, rgba_ptr, stream_ptr=None, stats_ptr=None, comp_ids_ptr=None,
-                      n_comp_ptr=None, comp_cap=0):
+                      n_comp_ptr=None, comp_cap=0, antialiasing=False):
         """Asynchronous traceRays into caller-owned device memory (GT_BUF_DEVICE),
-        optionally writing the backward checkpoint."""
+        optionally writing the backward checkpoint (or, with antialiasing,
+        averaging the jittered frame into rgba)."""
         f = Frame()
         f.rgba = rgba_ptr
+        f.antialiasing = 1 if antialiasing else 0
         f.stats = stats_ptr
         f.comp_ids, f.n_comp, f.comp_cap = comp_ids_ptr, n_comp_ptr, comp_cap
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 1, stream_ptr))
@@ -400,6 +410,7 @@ class Batch:
             F[i].n_rounds = fr.get("n_rounds")
             F[i].T = fr.get("T")
             F[i].max_rounds = fr.get("max_rounds", 0)
+            F[i].antialiasing = 1 if fr.get("antialiasing") else 0
         F[0].stats = stats_ptr
         self._scene = scene  # keeps the scene alive
         self._h = C.c_void_p()

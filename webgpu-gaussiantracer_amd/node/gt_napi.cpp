@@ -86,6 +86,18 @@ double num_prop(napi_env env, napi_value obj, const char* name, double dflt) {
   return d;
 }
 
+bool bool_prop(napi_env env, napi_value obj, const char* name, bool dflt) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return dflt;
+  napi_value v;
+  if (napi_get_named_property(env, obj, name, &v) != napi_ok) return dflt;
+  bool b = dflt;
+  napi_valuetype ty;
+  napi_typeof(env, v, &ty);
+  if (ty == napi_boolean) napi_get_value_bool(env, v, &b);
+  return b;
+}
+
 void set_num(napi_env env, napi_value obj, const char* name, double d) {
   napi_value v;
   napi_create_double(env, d, &v);
@@ -317,14 +329,33 @@ napi_value js_render(napi_env env, napi_callback_info info) {
                    (float)num_prop(env, o, "alpha_min", 0.2), (uint32_t)num_prop(env, o, "hit_array_size", 6),
                    (uint32_t)num_prop(env, o, "sh_degree_max", 3), (int32_t)num_prop(env, o, "earlyStop", 0),
                    (uint32_t)num_prop(env, o, "debugMode", 0));
+  /* antialiasing (GS_shader.ts:282-307, compiled out in the reference):
+   * jittered rays; with accumulatedFrames > 0 the frame is averaged into
+   * opts.prev, the previous average */
+  const bool aa = bool_prop(env, o, "antialiasing", false);
+  u.accumulated_frames = (uint32_t)num_prop(env, o, "accumulatedFrames", 0);
   float* rgba = nullptr;
   napi_value arr = new_f32_array(env, (size_t)W * H * 4, &rgba);
   if (!arr) {
     napi_throw_error(env, nullptr, "allocation failed");
     return nullptr;
   }
+  if (aa && u.accumulated_frames) {
+    bool has = false;
+    napi_value pv;
+    float* prev = nullptr;
+    size_t plen = 0;
+    if (napi_has_named_property(env, o, "prev", &has) != napi_ok || !has ||
+        napi_get_named_property(env, o, "prev", &pv) != napi_ok || !get_f32_array(env, pv, &prev, &plen) ||
+        plen < (size_t)W * H * 4) {
+      napi_throw_type_error(env, nullptr, "antialiasing with accumulatedFrames > 0 needs opts.prev: Float32Array(W*H*4)");
+      return nullptr;
+    }
+    std::memcpy(rgba, prev, (size_t)W * H * 4 * sizeof(float));
+  }
   gt_frame f{};
   f.rgba = rgba;
+  f.antialiasing = aa ? 1u : 0u;
   const int rc = gt_render(s, &u, W, H, &f, GT_BUF_HOST, nullptr);
   if (rc != GT_OK) return throw_gt(env, rc);
   return arr;

@@ -14,6 +14,9 @@ export interface RenderOptions {
   t_max?: number;           // default 1e5
   sh_degree_max?: number;   // 0..3, default 3
   earlyStop?: number;       // rounds, <= 0: never
+  antialiasing?: boolean;   // the reference's compiled-out AA path: jittered rays (default false)
+  accumulatedFrames?: number; // frames already averaged (the uniform of main.ts:455, 597)
+  prev?: Float32Array;      // the previous average, required when accumulatedFrames > 0
 }
 
 export declare const DEFAULTS: Readonly<Required<Omit<RenderOptions, 'width' | 'height'>>>;
