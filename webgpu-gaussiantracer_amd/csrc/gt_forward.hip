@@ -225,7 +225,7 @@ __device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, floa
       const uint32_t ref = stk.pop();
       if (ref & GT_LEAF_BIT) {
         pfirst = ref & GT_LEAF_FIRST_MASK;
-        pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+        pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
         break;
       }
       st.nodes++;
@@ -301,7 +301,7 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
     const bool in = ((lane < 32u ? mlo >> lane : mhi >> (lane - 32u)) & 1u) != 0u;
     if (ref & GT_LEAF_BIT) {
       const uint32_t first = ref & GT_LEAF_FIRST_MASK;
-      const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+      const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
       /* uniform loop, lane predicate inside: a divergent loop around the
        * k-buffer updates made the compiler copy (and spill) the buffer */
       for (uint32_t q = 0; q < cnt; ++q)
@@ -808,7 +808,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_
         const uint32_t ref = stk.pop();
         if (ref & GT_LEAF_BIT) {
           pfirst = ref & GT_LEAF_FIRST_MASK;
-          pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & 7u) + 1u;
+          pcount = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
           break;
         }
         st.nodes++;

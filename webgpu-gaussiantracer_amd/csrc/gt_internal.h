@@ -20,11 +20,15 @@
 
 #define GT_NODE_EMPTY 0xffffffffu
 #define GT_LEAF_BIT 0x80000000u
-#ifndef GT_LEAF_MAX
-#define GT_LEAF_MAX 8u                /* proxies per leaf (at most 8: 3-bit count) */
+#ifndef GT_LEAF_COUNT_BITS
+#define GT_LEAF_COUNT_BITS 3          /* leaf ref: 1 | (count-1):BITS | first:(31-BITS) */
 #endif
-#define GT_LEAF_COUNT_SHIFT 28        /* leaf ref: 1 | (count-1):3 | first:28 */
-#define GT_LEAF_FIRST_MASK 0x0fffffffu
+#ifndef GT_LEAF_MAX
+#define GT_LEAF_MAX (1u << GT_LEAF_COUNT_BITS) /* proxies per leaf */
+#endif
+#define GT_LEAF_COUNT_SHIFT (31 - GT_LEAF_COUNT_BITS)
+#define GT_LEAF_COUNT_MASK ((1u << GT_LEAF_COUNT_BITS) - 1u)
+#define GT_LEAF_FIRST_MASK ((1u << GT_LEAF_COUNT_SHIFT) - 1u)
 #define GT_MAX_PROXIES GT_LEAF_FIRST_MASK
 #define GT_STACK_MAX 96               /* traversal stack entries per ray */
 /* SAH cost of a proxy test relative to a 4-wide node visit (host and
