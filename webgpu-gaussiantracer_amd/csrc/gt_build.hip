@@ -23,9 +23,12 @@
  *   k_karras   binary radix tree over the sorted keys (T. Karras, "Maximizing
  *              parallelism in the construction of BVHs, octrees, and k-d
  *              trees", HPG 2012), one thread per inner node
- *   k_refit    bottom-up boxes with one arrival counter per node (the second
+ *   k_refit_local / k_refit_top
+ *              bottom-up boxes with one arrival counter per node (the second
  *              child to arrive continues), and the SAH-optimal leaf decision
- *              cost = min(C_I n A, C_T A + cost_L + cost_R) for n <= 8
+ *              cost = min(C_I n A, C_T A + cost_L + cost_R) for n <= 8:
+ *              per block with LDS counters up to the nodes that span
+ *              blocks, then those in one workgroup (no device-scope fences)
  *   k_collapse top-down, one launch per level: each 4-wide node opens the
  *              inner child of largest surface area until it has four
  *              children (the rule of the host builder); subtrees marked as
@@ -54,7 +57,8 @@
 #define BIN_NONE 0xffffffffu
 
 /* counters (u32) */
-enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OUT = 8, CTL_OVF = 9, CTL_WORDS = 16 };
+enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OUT = 8, CTL_OVF = 9, CTL_ITEMS = 10,
+       CTL_WORDS = 16 };
 
 namespace {
 
@@ -287,46 +291,76 @@ __device__ __forceinline__ void child_data(const BNode* bn, const float4* lbox, 
   }
 }
 
-/* One thread per leaf climbs while it is the second child to arrive.  A
- * node whose leaves all belong to this block's threads is completed by
- * threads of this block only, so its arrival handshake needs workgroup
- * scope; only nodes whose range crosses a block boundary (a few percent)
- * pay the device-scope fences (an L2 write-back across XCDs). */
-__global__ __launch_bounds__(BUILD_BLOCK) void k_refit(uint32_t np, BNode* __restrict__ bn,
-                                                       const float4* __restrict__ lbox,
-                                                       const uint32_t* __restrict__ lparent, uint32_t* visit) {
+/* Completes inner node N from its two children: box union, and the
+ * SAH-optimal leaf decision cost = min(C_I n A, C_T A + cost_L + cost_R). */
+__device__ __forceinline__ void refit_node(BNode* __restrict__ bn, const float4* __restrict__ lbox, BNode* N) {
+  float llo[3], lhi[3], rlo[3], rhi[3];
+  double cl, cr;
+  child_data(bn, lbox, N->left, llo, lhi, &cl);
+  child_data(bn, lbox, N->right, rlo, rhi, &cr);
+  float lo[3], hi[3];
+  for (int j = 0; j < 3; ++j) {
+    lo[j] = fminf(llo[j], rlo[j]);
+    hi[j] = fmaxf(lhi[j], rhi[j]);
+    N->lo[j] = lo[j];
+    N->hi[j] = hi[j];
+  }
+  const double A = box_area(lo, hi);
+  const double split = 1.0 * A + cl + cr;
+  const double leafc = GT_SAH_ISECT * (double)N->count * A;
+  const bool leaf = N->count <= GT_LEAF_MAX && leafc <= split;
+  N->cost = leaf ? leafc : split;
+  N->leaf = leaf ? 1u : 0u;
+}
+
+/* Refit, phase 1: one thread per leaf climbs while it is the second child to
+ * arrive, through the nodes whose leaves all belong to this block (an inner
+ * node's index is an end of its range, so such a node's index is in the
+ * block's range too: its arrival counter lives in LDS and the handshake is
+ * workgroup-scoped).  Where the next node spans blocks the climb stops and
+ * the completed child goes to the list `items` for phase 2. */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_refit_local(uint32_t np, BNode* __restrict__ bn,
+                                                             const float4* __restrict__ lbox,
+                                                             const uint32_t* __restrict__ lparent,
+                                                             uint32_t* __restrict__ items, unsigned int* ctl) {
+  __shared__ uint32_t s_visit[BUILD_BLOCK];
+  s_visit[threadIdx.x] = 0u;
+  __syncthreads();
   const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
   if (p >= np) return;
-  const uint32_t blo = blockIdx.x * BUILD_BLOCK, bhi = min(blo + BUILD_BLOCK, np); /* this block's leaves */
-  uint32_t node = lparent[p];
+  const uint32_t blo = blockIdx.x * BUILD_BLOCK, bhi = min(blo + BUILD_BLOCK, np);
+  uint32_t child = BIN_LEAF | p, node = lparent[p];
   while (node != BIN_NONE) {
     BNode* N = bn + node;
-    const bool local = N->first >= blo && N->first + N->count <= bhi;
-    if (local) {
-      if (__hip_atomic_fetch_add(visit + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
-    } else {
-      __threadfence(); /* release this thread's writes below node */
-      if (atomicAdd(visit + node, 1u) == 0u) return;
-      __threadfence(); /* acquire the sibling's */
+    if (!(N->first >= blo && N->first + N->count <= bhi)) { /* spans blocks: phase 2 */
+      items[atomicAdd(ctl + CTL_ITEMS, 1u)] = child;
+      return;
     }
-    float llo[3], lhi[3], rlo[3], rhi[3];
-    double cl, cr;
-    child_data(bn, lbox, N->left, llo, lhi, &cl);
-    child_data(bn, lbox, N->right, rlo, rhi, &cr);
-    float lo[3], hi[3];
-    for (int j = 0; j < 3; ++j) {
-      lo[j] = fminf(llo[j], rlo[j]);
-      hi[j] = fmaxf(lhi[j], rhi[j]);
-      N->lo[j] = lo[j];
-      N->hi[j] = hi[j];
-    }
-    const double A = box_area(lo, hi);
-    const double split = 1.0 * A + cl + cr;
-    const double leafc = GT_SAH_ISECT * (double)N->count * A;
-    const bool leaf = N->count <= GT_LEAF_MAX && leafc <= split;
-    N->cost = leaf ? leafc : split;
-    N->leaf = leaf ? 1u : 0u;
+    if (__hip_atomic_fetch_add(&s_visit[node - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+      return;
+    refit_node(bn, lbox, N);
+    child = node;
     node = N->parent;
+  }
+}
+
+/* Refit, phase 2: one workgroup climbs from every phase-1 item through the
+ * nodes that span blocks (all that phase 1 wrote is visible: a kernel
+ * boundary), with workgroup-scoped arrival counters in `visit`. */
+__global__ __launch_bounds__(1024) void k_refit_top(BNode* __restrict__ bn, const float4* __restrict__ lbox,
+                                                    const uint32_t* __restrict__ lparent,
+                                                    const uint32_t* __restrict__ items, const unsigned int* ctl,
+                                                    uint32_t* visit) {
+  const uint32_t n_items = ctl[CTL_ITEMS];
+  for (uint32_t it = threadIdx.x; it < n_items; it += 1024u) {
+    const uint32_t c = items[it];
+    uint32_t node = (c & BIN_LEAF) ? lparent[c & ~BIN_LEAF] : bn[c].parent;
+    while (node != BIN_NONE) {
+      if (__hip_atomic_fetch_add(visit + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
+      BNode* N = bn + node;
+      refit_node(bn, lbox, N);
+      node = N->parent;
+    }
   }
 }
 
@@ -431,7 +465,7 @@ inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + BUILD_BLOCK - 1)
 }  // namespace
 
 struct gt_devbuild {
-  DevBuf o2w, w2o, box, keys_a, keys_b, sort_tmp, lbox, bnode, lparent, visit, front_a, front_b, need, ctl;
+  DevBuf o2w, w2o, box, keys_a, keys_b, sort_tmp, lbox, bnode, lparent, visit, items, front_a, front_b, need, ctl;
   unsigned int* host_ctl = nullptr; /* pinned, CTL_WORDS */
   ~gt_devbuild() {
     if (host_ctl) (void)hipHostFree(host_ctl);
@@ -513,6 +547,7 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
     GT_TRY(gt_devbuf_reserve(w->bnode, (size_t)np * sizeof(BNode)));
     GT_TRY(gt_devbuf_reserve(w->lparent, (size_t)np * 4));
     GT_TRY(gt_devbuf_reserve(w->visit, (size_t)np * 4));
+    GT_TRY(gt_devbuf_reserve(w->items, (size_t)np * 4));
     GT_TRY(gt_devbuf_reserve(w->front_a, (size_t)cap * sizeof(Front)));
     GT_TRY(gt_devbuf_reserve(w->front_b, (size_t)cap * sizeof(Front)));
     GT_TRY(gt_devbuf_reserve(w->need, (size_t)cap * 4));
@@ -528,8 +563,11 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
       k_karras<<<blocks_for(np - 1), BUILD_BLOCK, 0, st>>>((const unsigned long long*)w->keys_b.p, (int)np, bn,
                                                            (uint32_t*)w->lparent.p);
       HIP_TRY(hipGetLastError());
-      k_refit<<<blocks_for(np), BUILD_BLOCK, 0, st>>>(np, bn, lbox, (const uint32_t*)w->lparent.p,
-                                                      (uint32_t*)w->visit.p);
+      k_refit_local<<<blocks_for(np), BUILD_BLOCK, 0, st>>>(np, bn, lbox, (const uint32_t*)w->lparent.p,
+                                                            (uint32_t*)w->items.p, ctl);
+      HIP_TRY(hipGetLastError());
+      k_refit_top<<<1, 1024, 0, st>>>(bn, lbox, (const uint32_t*)w->lparent.p, (const uint32_t*)w->items.p, ctl,
+                                      (uint32_t*)w->visit.p);
       HIP_TRY(hipGetLastError());
     }
     /* top-down collapse, one launch per level; level L's nodes are
