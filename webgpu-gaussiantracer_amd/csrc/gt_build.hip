@@ -14,8 +14,9 @@
  *   k_prep     one thread per Gaussian: activations in double, the 48-B
  *              geometry and 192-B SH records, proxy transform + inverse and
  *              padded world box (gt_prep.h, the same code as the host
- *              builder, so the records are bit-identical), proxy count and
- *              centroid bounds (wave reductions + atomics)
+ *              builder, so the records are bit-identical), and per block
+ *              the proxy count and centroid bounds (wave + LDS reductions)
+ *   k_bounds   one workgroup folds the block partials into the totals
  *   k_keys     64-bit key = 30-bit Morton code of the box centroid << 32 |
  *              Gaussian id (unique keys; non-proxies sort last)
  *   radix sort rocPRIM radix_sort_keys
@@ -23,14 +24,17 @@
  *   k_karras   binary radix tree over the sorted keys (T. Karras, "Maximizing
  *              parallelism in the construction of BVHs, octrees, and k-d
  *              trees", HPG 2012), one thread per inner node
- *   k_refit_local / k_refit_top
+ *   k_refit_local / k_refit_mid / k_refit_top
  *              bottom-up boxes with one arrival counter per node (the second
  *              child to arrive continues), and the SAH-optimal leaf decision
  *              cost = min(C_I n A, C_T A + cost_L + cost_R) for n <= 8:
- *              per block with LDS counters up to the nodes that span
- *              blocks, then those in one workgroup (no device-scope fences)
- *   k_collapse top-down, one launch per level: each 4-wide node opens the
- *              inner child of largest surface area until it has four
+ *              per block of 256 leaves with LDS counters, then per group
+ *              of 4096 leaves (LDS counters again), then the nodes that span
+ *              groups in one workgroup (no device-scope fences)
+ *   k_collapse top-down, one launch per level (the level sizes stay on
+ *              the device; the host reads them every 8 levels): each
+ *              4-wide node opens the inner child of largest surface area
+ *              until it has four
  *              children (the rule of the host builder); subtrees marked as
  *              leaves become leaf refs over their contiguous proxy range
  *   k_need     bottom-up worst-case traversal stack per node
@@ -56,9 +60,12 @@
 #define BIN_LEAF 0x80000000u /* binary ref: leaf = sorted proxy index | BIN_LEAF */
 #define BIN_NONE 0xffffffffu
 
-/* counters (u32) */
-enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OUT = 8, CTL_OVF = 9, CTL_ITEMS = 10,
-       CTL_WORDS = 16 };
+/* counters (u32); CTL_LVL + L = fronts (4-wide nodes) of collapse level L */
+enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CTL_ITEMS = 10, CTL_LVL = 16,
+       CTL_WORDS = CTL_LVL + GT_STACK_MAX + 4 };
+#define COLLAPSE_BATCH 8 /* collapse levels launched between host reads of the level counts */
+#define MID_LEAVES (16 * BUILD_BLOCK) /* leaves per refit phase-2 workgroup (LDS counters) */
+#define PART_WORDS 8 /* k_prep partial per block: count, centroid min[3], max[3], pad */
 
 namespace {
 
@@ -109,11 +116,11 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 }
 
 __global__ void k_init(unsigned int* ctl) {
-  const uint32_t i = threadIdx.x;
-  if (i < CTL_WORDS) {
+  for (uint32_t i = threadIdx.x; i < CTL_WORDS; i += blockDim.x) {
     uint32_t v = 0;
     if (i >= CTL_CMIN && i < CTL_CMIN + 3) v = 0xffffffffu;
-    if (i == CTL_NODES) v = 1u; /* node 0 = root */
+    if (i == CTL_NODES) v = 1u;   /* node 0 = root */
+    if (i == CTL_LVL) v = 1u;     /* level 0 = the root's front */
     ctl[i] = v;
   }
 }
@@ -127,7 +134,7 @@ __global__ void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
 __global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ raw, uint32_t n, double alpha_min,
                                                       float* __restrict__ geom, float* __restrict__ sh,
                                                       float* __restrict__ o2w, float* __restrict__ w2o,
-                                                      float4* __restrict__ box, unsigned int* __restrict__ ctl) {
+                                                      float4* __restrict__ box, uint32_t* __restrict__ part) {
   const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
   bool has = false;
   float c[3] = {0.0f, 0.0f, 0.0f};
@@ -176,13 +183,66 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ 
     mn[j] = wave_min_u32(has ? f2ord(c[j]) : 0xffffffffu);
     mx[j] = wave_max_u32(has ? f2ord(c[j]) : 0u);
   }
-  if (lane == 0 && m) {
-    atomicAdd(ctl + CTL_NPROX, (unsigned int)__popcll(m));
+  /* one partial per block, no atomics: 7 same-line atomics per wave took
+   * 1.26 ms at 1M Gaussians, serialised at one L2 channel */
+  __shared__ uint32_t s_part[BUILD_BLOCK / 64][PART_WORDS];
+  const uint32_t wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_part[wv][0] = (uint32_t)__popcll(m);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      atomicMin(ctl + CTL_CMIN + j, mn[j]);
-      atomicMax(ctl + CTL_CMAX + j, mx[j]);
+      s_part[wv][1 + j] = mn[j];
+      s_part[wv][4 + j] = mx[j];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const uint32_t j = threadIdx.x;
+    uint32_t v = s_part[0][j];
+#pragma unroll
+    for (uint32_t w = 1; w < BUILD_BLOCK / 64; ++w) {
+      const uint32_t x = s_part[w][j];
+      v = j == 0 ? v + x : (j < 4 ? min(v, x) : max(v, x));
+    }
+    part[(size_t)blockIdx.x * PART_WORDS + j] = v;
+  }
+}
+
+/* proxy count and centroid bounds of k_prep's per-block partials -> ctl */
+__global__ __launch_bounds__(1024) void k_bounds(const uint32_t* __restrict__ part, uint32_t nblocks,
+                                                 unsigned int* __restrict__ ctl) {
+  uint32_t v[7] = {0u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) {
+    const uint32_t* P = part + (size_t)b * PART_WORDS;
+    v[0] += P[0];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      v[1 + j] = min(v[1 + j], P[1 + j]);
+      v[4 + j] = max(v[4 + j], P[4 + j]);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v[0] += (uint32_t)__shfl_xor((int)v[0], o, 64);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    v[1 + j] = wave_min_u32(v[1 + j]);
+    v[4 + j] = wave_max_u32(v[4 + j]);
+  }
+  __shared__ uint32_t s_red[1024 / 64][PART_WORDS];
+  const uint32_t wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) s_red[wv][j] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const uint32_t j = threadIdx.x;
+    uint32_t r = s_red[0][j];
+    for (uint32_t w = 1; w < 1024 / 64; ++w) {
+      const uint32_t x = s_red[w][j];
+      r = j == 0 ? r + x : (j < 4 ? min(r, x) : max(r, x));
+    }
+    ctl[CTL_NPROX + j] = r; /* NPROX, CMIN[3], CMAX[3] are words 0..6 */
   }
 }
 
@@ -318,35 +378,94 @@ __device__ __forceinline__ void refit_node(BNode* __restrict__ bn, const float4*
  * node's index is an end of its range, so such a node's index is in the
  * block's range too: its arrival counter lives in LDS and the handshake is
  * workgroup-scoped).  Where the next node spans blocks the climb stops and
- * the completed child goes to the list `items` for phase 2. */
+ * the completed child is handed to phase 2, in the list of its group of
+ * MID_LEAVES leaves.  The handed-over subtrees are disjoint and lie in the
+ * block, so a block hands over at most BUILD_BLOCK of them and a group at
+ * most MID_LEAVES; one global atomic per block reserves the block's slots. */
 __global__ __launch_bounds__(BUILD_BLOCK) void k_refit_local(uint32_t np, BNode* __restrict__ bn,
                                                              const float4* __restrict__ lbox,
                                                              const uint32_t* __restrict__ lparent,
-                                                             uint32_t* __restrict__ items, unsigned int* ctl) {
+                                                             uint32_t* __restrict__ items,
+                                                             uint32_t* __restrict__ mid_count) {
   __shared__ uint32_t s_visit[BUILD_BLOCK];
+  __shared__ uint32_t s_out[BUILD_BLOCK];
+  __shared__ uint32_t s_n, s_base;
   s_visit[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) s_n = 0u;
   __syncthreads();
   const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
-  if (p >= np) return;
   const uint32_t blo = blockIdx.x * BUILD_BLOCK, bhi = min(blo + BUILD_BLOCK, np);
-  uint32_t child = BIN_LEAF | p, node = lparent[p];
-  while (node != BIN_NONE) {
-    BNode* N = bn + node;
-    if (!(N->first >= blo && N->first + N->count <= bhi)) { /* spans blocks: phase 2 */
-      items[atomicAdd(ctl + CTL_ITEMS, 1u)] = child;
-      return;
+  if (p < np) {
+    uint32_t child = BIN_LEAF | p, node = lparent[p];
+    while (node != BIN_NONE) {
+      BNode* N = bn + node;
+      if (!(N->first >= blo && N->first + N->count <= bhi)) { /* spans blocks: phase 2 */
+        s_out[atomicAdd(&s_n, 1u)] = child;
+        break;
+      }
+      if (__hip_atomic_fetch_add(&s_visit[node - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        break;
+      refit_node(bn, lbox, N);
+      child = node;
+      node = N->parent;
     }
-    if (__hip_atomic_fetch_add(&s_visit[node - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
-      return;
-    refit_node(bn, lbox, N);
-    child = node;
-    node = N->parent;
   }
+  __syncthreads();
+  const uint32_t m = s_n;
+  if (m == 0u) return;
+  const uint32_t g = blockIdx.x / (MID_LEAVES / BUILD_BLOCK);
+  if (threadIdx.x == 0) s_base = atomicAdd(mid_count + g, m);
+  __syncthreads();
+  if (threadIdx.x < m) items[(size_t)g * MID_LEAVES + s_base + threadIdx.x] = s_out[threadIdx.x];
 }
 
-/* Refit, phase 2: one workgroup climbs from every phase-1 item through the
- * nodes that span blocks (all that phase 1 wrote is visible: a kernel
- * boundary), with workgroup-scoped arrival counters in `visit`. */
+/* Refit, phase 2: one workgroup per group of MID_LEAVES leaves climbs from
+ * the group's phase-1 items through the nodes that lie in the group (LDS
+ * arrival counters, workgroup scope; phase 1's writes are visible across the
+ * kernel boundary).  Children of nodes that span groups go to phase 3 (at
+ * most MID_LEAVES per group, by the same disjointness). */
+__global__ __launch_bounds__(1024) void k_refit_mid(uint32_t np, BNode* __restrict__ bn,
+                                                    const float4* __restrict__ lbox,
+                                                    const uint32_t* __restrict__ lparent,
+                                                    const uint32_t* __restrict__ items,
+                                                    const uint32_t* __restrict__ mid_count,
+                                                    uint32_t* __restrict__ top_items, unsigned int* ctl) {
+  __shared__ uint32_t s_visit[MID_LEAVES];
+  __shared__ uint32_t s_out[MID_LEAVES];
+  __shared__ uint32_t s_n, s_base;
+  for (uint32_t i = threadIdx.x; i < MID_LEAVES; i += 1024u) s_visit[i] = 0u;
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  const uint32_t glo = blockIdx.x * MID_LEAVES, ghi = min(glo + MID_LEAVES, np);
+  const uint32_t cnt = mid_count[blockIdx.x];
+  const uint32_t* in = items + (size_t)blockIdx.x * MID_LEAVES;
+  for (uint32_t it = threadIdx.x; it < cnt; it += 1024u) {
+    uint32_t child = in[it];
+    uint32_t node = (child & BIN_LEAF) ? lparent[child & ~BIN_LEAF] : bn[child].parent;
+    while (node != BIN_NONE) {
+      BNode* N = bn + node;
+      if (!(N->first >= glo && N->first + N->count <= ghi)) { /* spans groups: phase 3 */
+        s_out[atomicAdd(&s_n, 1u)] = child;
+        break;
+      }
+      if (__hip_atomic_fetch_add(&s_visit[node - glo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        break;
+      refit_node(bn, lbox, N);
+      child = node;
+      node = N->parent;
+    }
+  }
+  __syncthreads();
+  const uint32_t m = s_n;
+  if (m == 0u) return;
+  if (threadIdx.x == 0) s_base = atomicAdd(ctl + CTL_ITEMS, m);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < m; i += 1024u) top_items[s_base + i] = s_out[i];
+}
+
+/* Refit, phase 3: one workgroup climbs from every phase-2 item through the
+ * nodes that span groups (all that phases 1 and 2 wrote is visible: kernel
+ * boundaries), with workgroup-scoped arrival counters in `visit`. */
 __global__ __launch_bounds__(1024) void k_refit_top(BNode* __restrict__ bn, const float4* __restrict__ lbox,
                                                     const uint32_t* __restrict__ lparent,
                                                     const uint32_t* __restrict__ items, const unsigned int* ctl,
@@ -369,14 +488,13 @@ __device__ __forceinline__ bool openable(const BNode* bn, uint32_t ref) {
 }
 
 /* one 4-wide node per work item; cap = capacity of nodes[] and out[] */
-__global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restrict__ in, uint32_t n_in,
-                                                          Front* __restrict__ out, unsigned int* ctl,
-                                                          const BNode* __restrict__ bn,
-                                                          const float4* __restrict__ lbox,
-                                                          gt_node4* __restrict__ nodes, uint32_t cap) {
-  const uint32_t t = blockIdx.x * BUILD_BLOCK + threadIdx.x;
-  if (t >= n_in) return;
-  const Front e = in[t];
+/* Collapse level L: the input count is ctl[CTL_LVL + L] (written by the
+ * launch of level L - 1), so the host launches levels without reading it;
+ * every inner child becomes a 4-wide node of level L + 1, allocated from
+ * CTL_NODES, which makes each level's nodes one contiguous index range. */
+__device__ void collapse_one(const Front e, Front* __restrict__ out, unsigned int* ctl, uint32_t level,
+                             const BNode* __restrict__ bn, const float4* __restrict__ lbox,
+                             gt_node4* __restrict__ nodes, uint32_t cap) {
   uint32_t ch[4];
   int m;
   if (!openable(bn, e.bref)) {
@@ -426,7 +544,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restric
         ref = dev_leaf_ref(b.first, b.count);
       } else {
         const uint32_t j = atomicAdd(ctl + CTL_NODES, 1u);
-        const uint32_t slot = atomicAdd(ctl + CTL_OUT, 1u);
+        const uint32_t slot = atomicAdd(ctl + CTL_LVL + level + 1u, 1u);
         if (j < cap && slot < cap) {
           out[slot] = Front{r, j};
           ref = j;
@@ -442,6 +560,16 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restric
     nd.child[c] = ref;
   }
   if (e.node4 < cap) nodes[e.node4] = nd;
+}
+
+__global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restrict__ in, uint32_t level,
+                                                          Front* __restrict__ out, unsigned int* ctl,
+                                                          const BNode* __restrict__ bn,
+                                                          const float4* __restrict__ lbox,
+                                                          gt_node4* __restrict__ nodes, uint32_t cap) {
+  const uint32_t n_in = min(ctl[CTL_LVL + level], cap);
+  for (uint32_t t = blockIdx.x * BUILD_BLOCK + threadIdx.x; t < n_in; t += gridDim.x * BUILD_BLOCK)
+    collapse_one(in[t], out, ctl, level, bn, lbox, nodes, cap);
 }
 
 /* worst-case stack of the subtree: max(m, (m - 1) + max need of inner children) */
@@ -465,7 +593,8 @@ inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + BUILD_BLOCK - 1)
 }  // namespace
 
 struct gt_devbuild {
-  DevBuf o2w, w2o, box, keys_a, keys_b, sort_tmp, lbox, bnode, lparent, visit, items, front_a, front_b, need, ctl;
+  DevBuf o2w, w2o, box, part, keys_a, keys_b, sort_tmp, lbox, bnode, lparent, visit, items, mid_count, top_items,
+      front_a, front_b, need, ctl;
   unsigned int* host_ctl = nullptr; /* pinned, CTL_WORDS */
   ~gt_devbuild() {
     if (host_ctl) (void)hipHostFree(host_ctl);
@@ -509,16 +638,19 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
   GT_TRY(gt_devbuf_reserve(w->o2w, (size_t)n * 48));
   GT_TRY(gt_devbuf_reserve(w->w2o, (size_t)n * 48));
   GT_TRY(gt_devbuf_reserve(w->box, (size_t)n * 32));
+  GT_TRY(gt_devbuf_reserve(w->part, (size_t)blocks_for(n) * PART_WORDS * 4));
   GT_TRY(gt_devbuf_reserve(w->keys_a, (size_t)n * 8));
   GT_TRY(gt_devbuf_reserve(w->keys_b, (size_t)n * 8));
   GT_TRY(gt_devbuf_reserve(w->ctl, CTL_WORDS * sizeof(unsigned int)));
   unsigned int* ctl = (unsigned int*)w->ctl.p;
-  k_init<<<1, 64, 0, st>>>(ctl);
+  k_init<<<1, BUILD_BLOCK, 0, st>>>(ctl);
   HIP_TRY(hipGetLastError());
   if (n > 0) {
     k_prep<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float*)s->raw_dev.p, n, s->alpha_min, (float*)s->geom.p,
                                                   (float*)s->sh.p, (float*)w->o2w.p, (float*)w->w2o.p,
-                                                  (float4*)w->box.p, ctl);
+                                                  (float4*)w->box.p, (uint32_t*)w->part.p);
+    HIP_TRY(hipGetLastError());
+    k_bounds<<<1, 1024, 0, st>>>((const uint32_t*)w->part.p, blocks_for(n), ctl);
     HIP_TRY(hipGetLastError());
     k_keys<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float4*)w->box.p, n, ctl,
                                                   (unsigned long long*)w->keys_a.p);
@@ -547,7 +679,10 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
     GT_TRY(gt_devbuf_reserve(w->bnode, (size_t)np * sizeof(BNode)));
     GT_TRY(gt_devbuf_reserve(w->lparent, (size_t)np * 4));
     GT_TRY(gt_devbuf_reserve(w->visit, (size_t)np * 4));
-    GT_TRY(gt_devbuf_reserve(w->items, (size_t)np * 4));
+    const uint32_t n_groups = (uint32_t)((np + MID_LEAVES - 1) / MID_LEAVES);
+    GT_TRY(gt_devbuf_reserve(w->items, (size_t)n_groups * MID_LEAVES * 4));
+    GT_TRY(gt_devbuf_reserve(w->mid_count, (size_t)n_groups * 4));
+    GT_TRY(gt_devbuf_reserve(w->top_items, (size_t)np * 4));
     GT_TRY(gt_devbuf_reserve(w->front_a, (size_t)cap * sizeof(Front)));
     GT_TRY(gt_devbuf_reserve(w->front_b, (size_t)cap * sizeof(Front)));
     GT_TRY(gt_devbuf_reserve(w->need, (size_t)cap * 4));
@@ -560,40 +695,54 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
     if (np >= 2) {
       k_fill<<<blocks_for(np), BUILD_BLOCK, 0, st>>>((uint32_t*)w->visit.p, np, 0u);
       HIP_TRY(hipGetLastError());
+      k_fill<<<blocks_for(n_groups), BUILD_BLOCK, 0, st>>>((uint32_t*)w->mid_count.p, n_groups, 0u);
+      HIP_TRY(hipGetLastError());
       k_karras<<<blocks_for(np - 1), BUILD_BLOCK, 0, st>>>((const unsigned long long*)w->keys_b.p, (int)np, bn,
                                                            (uint32_t*)w->lparent.p);
       HIP_TRY(hipGetLastError());
       k_refit_local<<<blocks_for(np), BUILD_BLOCK, 0, st>>>(np, bn, lbox, (const uint32_t*)w->lparent.p,
-                                                            (uint32_t*)w->items.p, ctl);
+                                                            (uint32_t*)w->items.p, (uint32_t*)w->mid_count.p);
       HIP_TRY(hipGetLastError());
-      k_refit_top<<<1, 1024, 0, st>>>(bn, lbox, (const uint32_t*)w->lparent.p, (const uint32_t*)w->items.p, ctl,
-                                      (uint32_t*)w->visit.p);
+      k_refit_mid<<<n_groups, 1024, 0, st>>>(np, bn, lbox, (const uint32_t*)w->lparent.p,
+                                             (const uint32_t*)w->items.p, (const uint32_t*)w->mid_count.p,
+                                             (uint32_t*)w->top_items.p, ctl);
+      HIP_TRY(hipGetLastError());
+      k_refit_top<<<1, 1024, 0, st>>>(bn, lbox, (const uint32_t*)w->lparent.p, (const uint32_t*)w->top_items.p,
+                                      ctl, (uint32_t*)w->visit.p);
       HIP_TRY(hipGetLastError());
     }
-    /* top-down collapse, one launch per level; level L's nodes are
-     * [level[L], level[L+1]) because a level's nodes are all allocated by
-     * the launch of the level above */
+    /* top-down collapse, one launch per level, COLLAPSE_BATCH levels between
+     * host reads; level L's nodes are [level[L], level[L + 1]) because a
+     * level's nodes are all allocated by the launch of the level above */
+    const uint32_t cgrid = std::min(blocks_for(cap), 1024u);
     std::vector<uint32_t> level{0u, 1u};
-    Front* fin = (Front*)w->front_a.p;
-    Front* fout = (Front*)w->front_b.p;
-    uint32_t n_in = 1;
-    while (n_in > 0) {
-      if (level.size() > GT_STACK_MAX + 2) {
-        gt_set_error("device build: BVH deeper than %u levels", GT_STACK_MAX);
-        return GT_E_LIMIT;
+    uint32_t launched = 0;
+    for (;;) {
+      for (uint32_t b = 0; b < COLLAPSE_BATCH && launched <= GT_STACK_MAX; ++b, ++launched) {
+        Front* fin = (Front*)((launched & 1u) ? w->front_b.p : w->front_a.p);
+        Front* fout = (Front*)((launched & 1u) ? w->front_a.p : w->front_b.p);
+        k_collapse<<<cgrid, BUILD_BLOCK, 0, st>>>(fin, launched, fout, ctl, bn, lbox, (gt_node4*)s->nodes.p, cap);
+        HIP_TRY(hipGetLastError());
       }
-      k_fill<<<1, BUILD_BLOCK, 0, st>>>(ctl + CTL_OUT, 1u, 0u);
-      HIP_TRY(hipGetLastError());
-      k_collapse<<<blocks_for(n_in), BUILD_BLOCK, 0, st>>>(fin, n_in, fout, ctl, bn, lbox, (gt_node4*)s->nodes.p, cap);
-      HIP_TRY(hipGetLastError());
       GT_TRY(read_ctl(w, st));
       if (w->host_ctl[CTL_OVF]) {
         gt_set_error("device build: more than %u BVH nodes", cap);
         return GT_E_LIMIT;
       }
-      n_in = w->host_ctl[CTL_OUT];
-      level.push_back(w->host_ctl[CTL_NODES]);
-      std::swap(fin, fout);
+      level.assign({0u, 1u});
+      uint32_t L = 1;
+      while (L <= launched && w->host_ctl[CTL_LVL + L] > 0u) {
+        level.push_back(level.back() + w->host_ctl[CTL_LVL + L]);
+        ++L;
+      }
+      if (L <= launched) { /* level L is empty: done */
+        level.push_back(level.back());
+        break;
+      }
+      if (launched > GT_STACK_MAX) {
+        gt_set_error("device build: BVH deeper than %u levels", GT_STACK_MAX);
+        return GT_E_LIMIT;
+      }
     }
     n_nodes = level.back();
     depth = (uint32_t)level.size() - 2u;
