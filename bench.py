@@ -25,8 +25,9 @@ c5: 3M Gaussians, 16 views at 1280x720, forward + recompute-mode backward
 per view with gradients accumulated across views, peak device memory.
 device_build: the same 1M-Gaussian scene built on the GPU from device
 parameters (rebuild time) and the forward frame time on that tree.
-train_c4: one training step (forward with checkpoint, backward, Adam, GPU
-rebuild) on the 200k-Gaussian 800x800 workload.
+train_c4: one training step (forward with checkpoint, backward, Adam, scene
+update) on the 200k-Gaussian 800x800 workload: refit of the host-built tree
+(the default), and a GPU rebuild for comparison.
 
 cpu_baseline: the oracle (the project's C restatement of the reference path,
 BVH variant, all host threads up to 16) on a seeded 1/3 sample of the rays.
@@ -358,7 +359,10 @@ def bench_device_build(gt, torch, dev, sp, raw, u, W, H, steps):
 def bench_train_c4(gt, torch, dev, sp, steps):
     """One training step at 200k single-cluster Gaussians, one 800x800 view
     (the forward + backward workload above): forward with checkpoint,
-    dL/dRGBA, backward, Adam with the 3DGS learning rates, GPU rebuild."""
+    dL/dRGBA, backward, Adam with the 3DGS learning rates, then the scene
+    update.  Measured twice: the default trainer (host binned-SAH tree built
+    once, refit after each step) and the rebuild trainer (GPU Morton tree
+    rebuilt after each step).  Both give the same renders and gradients."""
     import numpy as np
     from gtrace.train import Trainer
     raw = gt.generate_raw(seed=0, config="c4")
@@ -373,21 +377,33 @@ def bench_train_c4(gt, torch, dev, sp, steps):
     ts.render_device(u, W, H, target.data_ptr(), sp)
     torch.cuda.synchronize()
     ts.close()
-    x = torch.from_numpy(raw).to(dev)
-    tr = Trainer(x, [(u, W, H, target)], alpha_min=0.01)
-    loss0 = tr.step()
-    tr.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    losses = [tr.step() for _ in range(steps)]
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) / steps * 1e3
-    build_ms = tr.scene.info()["build_ms"]
-    tr.close()
-    return {"workload": "200000 Gaussians single cluster, one 800x800 view, k 16; step = forward with checkpoint + "
-                        "dL/dRGBA + backward + Adam (3DGS learning rates) + GPU scene rebuild",
-            "ms_per_step": round(ms, 3), "mrays": round(W * H / ms / 1e3, 3), "rebuild_ms": round(build_ms, 3),
-            "loss_first": round(loss0, 3), "loss_last": round(losses[-1], 3), "steps": steps}
+
+    def run(build, refit, nsteps):
+        x = torch.from_numpy(raw).to(dev)
+        tr = Trainer(x, [(u, W, H, target)], alpha_min=0.01, build=build, refit=refit)
+        loss0 = tr.step()
+        tr.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = [tr.step() for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / nsteps * 1e3
+        r = {"ms_per_step": round(ms, 3), "mrays": round(W * H / ms / 1e3, 3),
+             "update_ms": round(tr.scene.info()["build_ms"], 3), "rebuilds": tr.rebuilds,
+             "loss_first": round(loss0, 3), "loss_last": round(losses[-1], 3), "steps": nsteps}
+        tr.close()
+        return r
+
+    out = {"workload": "200000 Gaussians single cluster, one 800x800 view, k 16; step = forward with checkpoint + "
+                       "dL/dRGBA + backward + Adam (3DGS learning rates) + scene update (host SAH tree built once, "
+                       "refit each step; fallback GPU rebuild when the proxy set changes)"}
+    out.update(run("host", True, steps))
+    rb = run("gpu", False, max(3, steps // 2))
+    rb["workload"] = "the same step with a GPU rebuild (Morton radix tree) instead of the refit"
+    out["gpu_rebuild"] = rb
+    out["note"] = ("update_ms is the wall time of the last update call, including the wait for the queued "
+                   "backward at its first host read")
+    return out
 
 
 def main():

@@ -118,6 +118,19 @@ int gt_scene_create_raw_gpu(const float* raw, uint32_t n, double alpha_min, gt_s
  * backward calls on the scene. */
 int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* hip_stream);
 
+/* Updates any scene (host- or GPU-built) from DEVICE parameters (same n, same
+ * alpha_min) and keeps its tree: records, proxy transforms and proxy boxes
+ * are recomputed, and the node boxes are refit level by level.  Which hits a
+ * ray finds does not depend on the tree, so renders equal those of a fresh
+ * build of the same parameters, bit for bit.  A host-built tree keeps its
+ * SAH quality, which a rebuild on the GPU (Morton order) would not.  If the
+ * set of Gaussians with a proxy changed, the scene is rebuilt on the GPU
+ * instead and *rebuilt (if not NULL) is set to 1, else 0.  The scene then
+ * holds device data only (gt_scene_get_host_data downloads it).  Returns
+ * when done (synchronises hip_stream).  No counterpart in the reference,
+ * which does not train (README.md:68-69). */
+int gt_scene_refit_device(gt_scene* s, const float* raw_dev, void* hip_stream, uint32_t* rebuilt);
+
 void gt_scene_destroy(gt_scene* s);
 
 typedef struct gt_scene_info {

@@ -61,7 +61,8 @@
 #define BIN_NONE 0xffffffffu
 
 /* counters (u32); CTL_LVL + L = fronts (4-wide nodes) of collapse level L */
-enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CTL_ITEMS = 10, CTL_LVL = 16,
+enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CTL_ITEMS = 10, CTL_STALE = 11,
+       CTL_LVL = 16,
        CTL_WORDS = CTL_LVL + GT_STACK_MAX + 4 };
 #define COLLAPSE_BATCH 8 /* collapse levels launched between host reads of the level counts */
 #define MID_LEAVES (16 * BUILD_BLOCK) /* leaves per refit phase-2 workgroup (LDS counters) */
@@ -573,6 +574,67 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_collapse(const Front* __restric
 }
 
 /* worst-case stack of the subtree: max(m, (m - 1) + max need of inner children) */
+/* Refit, proxies: records and leaf boxes in leaf order from k_prep's output.
+ * The tree stays valid only for the same proxy set: a proxy whose Gaussian
+ * lost its proxy, or a different proxy count, marks the scene stale. */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_refit_prox(uint32_t np, float4* __restrict__ prox,
+                                                            const float4* __restrict__ w2o,
+                                                            const float4* __restrict__ box,
+                                                            float4* __restrict__ lbox, unsigned int* ctl) {
+  const uint32_t p = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (p == 0 && ctl[CTL_NPROX] != np) atomicOr(ctl + CTL_STALE, 1u);
+  if (p >= np) return;
+  float4* P = prox + 4 * (size_t)p;
+  const uint32_t id = __float_as_uint(P[3].x);
+  const float4 b0 = box[2 * (size_t)id], b1 = box[2 * (size_t)id + 1];
+  if (b0.w == 0.0f) {
+    atomicOr(ctl + CTL_STALE, 1u);
+    return;
+  }
+  const float4* V = w2o + 3 * (size_t)id;
+  P[0] = V[0];
+  P[1] = V[1];
+  P[2] = V[2];
+  lbox[2 * (size_t)p] = b0;
+  lbox[2 * (size_t)p + 1] = b1;
+}
+
+/* Refit, nodes of one level (deepest level first): each child slot's box is
+ * the union of its leaf's proxy boxes or of its child node's four slots
+ * (empty slots hold +inf / -inf and drop out of the union). */
+__global__ __launch_bounds__(BUILD_BLOCK) void k_refit4(gt_node4* __restrict__ nodes,
+                                                        const uint32_t* __restrict__ order, uint32_t first,
+                                                        uint32_t last, const float4* __restrict__ lbox) {
+  const uint32_t t = first + blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  if (t >= last) return;
+  const uint32_t j = order ? order[t] : t;
+  gt_node4 nd = nodes[j];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t r = nd.child[c];
+    if (r == GT_NODE_EMPTY) continue;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (r & GT_LEAF_BIT) {
+      const uint32_t f = r & GT_LEAF_FIRST_MASK, cnt = ((r >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
+      for (uint32_t q = f; q < f + cnt; ++q) {
+        const float4 a = lbox[2 * (size_t)q], b = lbox[2 * (size_t)q + 1];
+        lo[0] = fminf(lo[0], a.x); lo[1] = fminf(lo[1], a.y); lo[2] = fminf(lo[2], a.z);
+        hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
+      }
+    } else {
+      const gt_node4& k = nodes[r];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        lo[0] = fminf(lo[0], k.lo_x[q]); lo[1] = fminf(lo[1], k.lo_y[q]); lo[2] = fminf(lo[2], k.lo_z[q]);
+        hi[0] = fmaxf(hi[0], k.hi_x[q]); hi[1] = fmaxf(hi[1], k.hi_y[q]); hi[2] = fmaxf(hi[2], k.hi_z[q]);
+      }
+    }
+    nd.lo_x[c] = lo[0]; nd.lo_y[c] = lo[1]; nd.lo_z[c] = lo[2];
+    nd.hi_x[c] = hi[0]; nd.hi_y[c] = hi[1]; nd.hi_z[c] = hi[2];
+  }
+  nodes[j] = nd;
+}
+
 __global__ __launch_bounds__(BUILD_BLOCK) void k_need(const gt_node4* __restrict__ nodes, uint32_t first,
                                                       uint32_t last, uint32_t* __restrict__ need) {
   const uint32_t j = first + blockIdx.x * BUILD_BLOCK + threadIdx.x;
@@ -746,6 +808,7 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
     }
     n_nodes = level.back();
     depth = (uint32_t)level.size() - 2u;
+    s->rlevels.assign(level.begin(), level.end() - 1);
     uint32_t* needp = (uint32_t*)w->need.p;
     for (size_t L = level.size() - 2; L-- > 0;) {
       const uint32_t a = level[L], b = level[L + 1];
@@ -775,6 +838,103 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
   s->n_nodes = n_nodes;
   s->max_stack = need;
   s->depth = depth;
+  if (np == 0) s->rlevels.clear();
+  s->ridentity = true;
+  s->rsched = true;
+  s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GT_OK;
+}
+
+/* Refit schedule of a tree built elsewhere (the host builder numbers nodes
+ * depth first): breadth-first levels from the root over a host copy. */
+static int refit_schedule(gt_scene* s) {
+  const uint32_t nn = s->dev.n_nodes;
+  std::vector<gt_node4> nodes(nn);
+  if (nn) HIP_TRY(hipMemcpy(nodes.data(), s->nodes.p, (size_t)nn * sizeof(gt_node4), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> order;
+  order.reserve(nn);
+  s->rlevels.assign(1, 0u);
+  if (nn && s->dev.root != GT_NODE_EMPTY) {
+    order.push_back(s->dev.root);
+    size_t a = 0;
+    while (a < order.size()) {
+      const size_t b = order.size();
+      for (size_t t = a; t < b; ++t)
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t r = nodes[order[t]].child[c];
+          if (r == GT_NODE_EMPTY || (r & GT_LEAF_BIT)) continue;
+          if (r >= nn || order.size() >= nn) {
+            gt_set_error("refit: malformed BVH (child %u of %u nodes)", r, nn);
+            return GT_E_INVALID;
+          }
+          order.push_back(r);
+        }
+      s->rlevels.push_back((uint32_t)b);
+      a = b;
+    }
+  }
+  GT_TRY(gt_devbuf_reserve(s->rorder, std::max<size_t>(order.size(), 1) * 4));
+  if (!order.empty()) HIP_TRY(hipMemcpy(s->rorder.p, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+  s->ridentity = false;
+  s->rsched = true;
+  return GT_OK;
+}
+
+int gt_device_refit(gt_scene* s, const float* raw_dev, hipStream_t st, uint32_t* rebuilt) {
+  const auto t0 = std::chrono::steady_clock::now();
+  *rebuilt = 0;
+  if (!s->db) {
+    s->db = new (std::nothrow) gt_devbuild();
+    if (!s->db) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
+  }
+  gt_devbuild* w = s->db;
+  if (!w->host_ctl) HIP_TRY(hipHostMalloc((void**)&w->host_ctl, CTL_WORDS * sizeof(unsigned int)));
+  const uint32_t n = s->n, np = s->dev.n_prox;
+  if (!s->rsched) GT_TRY(refit_schedule(s));
+  const size_t raw_bytes = (size_t)n * GT_RAW_FLOATS * sizeof(float);
+  GT_TRY(gt_devbuf_reserve(s->raw_dev, raw_bytes));
+  if (raw_bytes && raw_dev != s->raw_dev.p)
+    HIP_TRY(hipMemcpyAsync(s->raw_dev.p, raw_dev, raw_bytes, hipMemcpyDeviceToDevice, st));
+  GT_TRY(gt_devbuf_reserve(s->geom, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(s->sh, (size_t)n * 192));
+  GT_TRY(gt_devbuf_reserve(w->o2w, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(w->w2o, (size_t)n * 48));
+  GT_TRY(gt_devbuf_reserve(w->box, (size_t)n * 32));
+  GT_TRY(gt_devbuf_reserve(w->part, (size_t)blocks_for(n) * PART_WORDS * 4));
+  GT_TRY(gt_devbuf_reserve(w->lbox, (size_t)np * 32));
+  GT_TRY(gt_devbuf_reserve(w->ctl, CTL_WORDS * sizeof(unsigned int)));
+  unsigned int* ctl = (unsigned int*)w->ctl.p;
+  k_init<<<1, BUILD_BLOCK, 0, st>>>(ctl);
+  HIP_TRY(hipGetLastError());
+  if (n > 0) {
+    k_prep<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float*)s->raw_dev.p, n, s->alpha_min, (float*)s->geom.p,
+                                                  (float*)s->sh.p, (float*)w->o2w.p, (float*)w->w2o.p,
+                                                  (float4*)w->box.p, (uint32_t*)w->part.p);
+    HIP_TRY(hipGetLastError());
+    k_bounds<<<1, 1024, 0, st>>>((const uint32_t*)w->part.p, blocks_for(n), ctl);
+    HIP_TRY(hipGetLastError());
+  }
+  k_refit_prox<<<std::max(blocks_for(np), 1u), BUILD_BLOCK, 0, st>>>(np, (float4*)s->prox.p,
+                                                                    (const float4*)w->w2o.p,
+                                                                    (const float4*)w->box.p,
+                                                                    (float4*)w->lbox.p, ctl);
+  HIP_TRY(hipGetLastError());
+  const uint32_t* order = s->ridentity ? nullptr : (const uint32_t*)s->rorder.p;
+  for (size_t L = s->rlevels.size(); L-- > 1;) {
+    const uint32_t a = s->rlevels[L - 1], b = s->rlevels[L];
+    if (b > a) {
+      k_refit4<<<blocks_for(b - a), BUILD_BLOCK, 0, st>>>((gt_node4*)s->nodes.p, order, a, b,
+                                                          (const float4*)w->lbox.p);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  GT_TRY(read_ctl(w, st));
+  s->device_built = true;
+  s->h = HostScene{};
+  if (w->host_ctl[CTL_STALE]) {
+    *rebuilt = 1;
+    return gt_device_build(s, st);
+  }
   s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GT_OK;
 }

@@ -431,6 +431,7 @@ static int build_and_upload(gt_scene* s) {
   s->n_nodes = (uint32_t)nodes.size();
   s->max_stack = need;
   s->depth = depth;
+  s->rsched = false; /* depth-first numbering: the refit schedule is made on first use */
   s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GT_OK;
 }
@@ -587,6 +588,15 @@ extern "C" int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* s
   }
   if (s->n && !raw_dev) { gt_set_error("gt_scene_update_device: raw_dev is NULL"); return GT_E_INVALID; }
   return device_rebuild(s, raw_dev, stream);
+}
+
+extern "C" int gt_scene_refit_device(gt_scene* s, const float* raw_dev, void* stream, uint32_t* rebuilt) {
+  if (!s) { gt_set_error("gt_scene_refit_device: scene is NULL"); return GT_E_INVALID; }
+  if (s->n && !raw_dev) { gt_set_error("gt_scene_refit_device: raw_dev is NULL"); return GT_E_INVALID; }
+  uint32_t r = 0;
+  const int rc = gt_device_refit(s, raw_dev, (hipStream_t)stream, &r);
+  if (rebuilt) *rebuilt = r;
+  return rc;
 }
 
 extern "C" int gt_scene_get_info(const gt_scene* s, gt_scene_info* info) {

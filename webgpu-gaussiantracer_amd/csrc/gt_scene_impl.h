@@ -54,6 +54,13 @@ struct gt_scene {
   gt_dev_scene dev{};
   uint32_t n_nodes = 0, max_stack = 0, depth = 0;
   double build_ms = 0.0;
+  /* refit schedule (gt_scene_refit_device): the nodes of level L are entries
+   * [rlevels[L], rlevels[L + 1]) of rorder, or those indices themselves when
+   * ridentity (the device builder numbers nodes level by level) */
+  std::vector<uint32_t> rlevels;
+  DevBuf rorder;
+  bool ridentity = false;
+  bool rsched = false; /* schedule valid for the current tree */
   ~gt_scene() { gt_devbuild_destroy(db); }
 };
 
@@ -65,6 +72,9 @@ int gt_scene_static_upload(gt_scene* s);
 
 /* gt_build.hip: builds every device array of s from s->raw_dev on the GPU */
 int gt_device_build(gt_scene* s, hipStream_t stream);
+/* gt_build.hip: new parameters, same tree topology (boxes refit); falls back
+ * to gt_device_build when the set of proxies changed (*rebuilt = 1) */
+int gt_device_refit(gt_scene* s, const float* raw_dev, hipStream_t stream, uint32_t* rebuilt);
 /* gt_build.hip: host copies of a device-built scene (gt_scene_get_host_data) */
 int gt_device_host_data(const gt_scene* s, float* packed, uint32_t* proxy_ids, float* o2w, float* w2o);
 

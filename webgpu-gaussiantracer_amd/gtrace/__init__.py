@@ -108,6 +108,7 @@ _lib.gt_scene_create_packed.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P
 _lib.gt_scene_set_alpha_min.argtypes = [_P, C.c_double]
 _lib.gt_scene_create_device.argtypes = [_P, C.c_uint32, C.c_double, _P, C.POINTER(_P)]
 _lib.gt_scene_update_device.argtypes = [_P, _P, _P]
+_lib.gt_scene_refit_device.argtypes = [_P, _P, _P, C.POINTER(C.c_uint32)]
 _lib.gt_scene_create_raw_gpu.argtypes = [_P, C.c_uint32, C.c_double, C.POINTER(_P)]
 _lib.gt_scene_destroy.argtypes = [_P]
 _lib.gt_scene_destroy.restype = None
@@ -137,7 +138,7 @@ _lib.gt_free.restype = None
 
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
-    "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_create_raw_gpu", "gt_scene_update_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
+    "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_create_raw_gpu", "gt_scene_update_device", "gt_scene_refit_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
     "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
 ]
@@ -309,6 +310,14 @@ class Scene:
     def update_device(self, raw_dev, stream_ptr=None):
         """Rebuilds a device-built scene from updated device parameters (same n)."""
         _check(_lib.gt_scene_update_device(self._h, raw_dev, stream_ptr))
+
+    def refit_device(self, raw_dev, stream_ptr=None):
+        """Updates the scene from device parameters (same n) keeping its tree
+        (boxes refit); renders equal a fresh build's.  Returns True when the
+        set of proxies changed and the scene was rebuilt on the GPU instead."""
+        rebuilt = C.c_uint32(0)
+        _check(_lib.gt_scene_refit_device(self._h, raw_dev, stream_ptr, C.byref(rebuilt)))
+        return bool(rebuilt.value)
 
     def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0):
         """Launch tuning (results do not depend on it); 0 = library default."""

@@ -7,7 +7,16 @@ loop the library's pieces make possible, with the parameters resident in HBM:
                     dL/dRGBA = 2 (render - target)      (loss = sum of squared RGB error)
                     gt_backward (accumulates dL/draw, n x 59)
     gt_adam_step    per-field learning rates
-    gt_scene_update_device   GPU rebuild of records, proxies and BVH
+    gt_scene_refit_device    records, proxies and the tree's boxes from the
+                    updated parameters (topology kept; a GPU rebuild when the
+                    set of proxies changed), or gt_scene_update_device, a
+                    full GPU rebuild
+
+Which hits a ray finds does not depend on the tree, so both updates give the
+same renders and gradients bit for bit.  They differ in speed: the refit is
+cheaper than a rebuild and keeps the tree of the first build, by default the
+host's binned-SAH tree, which renders faster than the GPU's Morton-order
+tree (by 20 % on a dense 200k-Gaussian cluster).
 
 Default learning rates are those of the 3D Gaussian Splatting reference
 optimiser (Kerbl et al. 2023): means 1.6e-4 x spatial scale, f_dc 2.5e-3,
@@ -40,7 +49,8 @@ class Trainer:
     images.  views: list of (uniforms, width, height, target) with target a
     CUDA float32 (height, width, 4) tensor (its alpha channel is ignored)."""
 
-    def __init__(self, raw, views, alpha_min=0.01, lr=None, betas=(0.9, 0.999), eps=1e-15, stream=None):
+    def __init__(self, raw, views, alpha_min=0.01, lr=None, betas=(0.9, 0.999), eps=1e-15, stream=None,
+                 build="host", refit=True):
         if not (raw.is_cuda and raw.dtype == torch.float32 and raw.is_contiguous() and raw.dim() == 2
                 and raw.shape[1] == GT_RAW_FLOATS):
             raise ValueError("raw must be a contiguous CUDA float32 tensor of shape (n, 59)")
@@ -52,7 +62,17 @@ class Trainer:
         self.betas, self.eps = betas, eps
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         sp = self.stream.cuda_stream
-        self.scene = Scene(raw_dev=raw.data_ptr(), n=self.n, alpha_min=self.alpha_min, stream_ptr=sp)
+        if build not in ("host", "gpu"):
+            raise ValueError("build must be 'host' (binned SAH) or 'gpu' (Morton radix tree)")
+        if build == "host" and not refit:
+            raise ValueError("a host-built tree is kept by refit; rebuilds (refit=False) need build='gpu'")
+        self.refit = bool(refit)
+        self.rebuilds = 0
+        if build == "host":
+            self.stream.synchronize()
+            self.scene = Scene(raw.detach().cpu().numpy(), alpha_min=self.alpha_min)
+        else:
+            self.scene = Scene(raw_dev=raw.data_ptr(), n=self.n, alpha_min=self.alpha_min, stream_ptr=sp)
         self.m = torch.zeros_like(raw)
         self.v = torch.zeros_like(raw)
         self.grads = torch.zeros_like(raw)
@@ -107,7 +127,10 @@ class Trainer:
             self.step_count += 1
             adam_step(self.raw.data_ptr(), self.grads.data_ptr(), self.m.data_ptr(), self.v.data_ptr(), self.n,
                       self.lr, self.step_count, self.betas[0], self.betas[1], self.eps, sp)
-            self.scene.update_device(self.raw.data_ptr(), sp)
+            if self.refit:
+                self.rebuilds += int(self.scene.refit_device(self.raw.data_ptr(), sp))
+            else:
+                self.scene.update_device(self.raw.data_ptr(), sp)
         if int(self.overflow.item()) != 0:
             raise RuntimeError("backward checkpoint overflowed")
         return float(loss.item())
