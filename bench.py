@@ -399,7 +399,7 @@ def bench_train_c4(gt, torch, dev, sp, steps):
                        "refit each step; fallback GPU rebuild when the proxy set changes)"}
     out.update(run("host", True, steps))
     rb = run("gpu", False, max(3, steps // 2))
-    rb["workload"] = "the same step with a GPU rebuild (Morton radix tree) instead of the refit"
+    rb["workload"] = "the same step with a GPU rebuild (radix tree with PLOC treelets) instead of the refit"
     out["gpu_rebuild"] = rb
     out["note"] = ("update_ms is the wall time of the last update call, including the wait for the queued "
                    "backward at its first host read")
