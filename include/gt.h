@@ -329,6 +329,26 @@ int gt_generate_scene(const gt_gen_params* g, float* raw /* n x 59 */);
 int gt_ply_read(const char* path, float** raw, uint32_t* n, uint32_t* sh_degree);
 void gt_free(void* p);
 
+/* ---- presentation: the screen pass and the Save button ---- */
+#define GT_TONEMAP_IDENTITY 0       /* what the reference's fragment shader returns: hdrColor (toneMapper.ts:22) */
+#define GT_TONEMAP_EXPOSURE 1       /* its unused ldrColor = 1 - exp(-1.66 hdr) (toneMapper.ts:19) */
+#define GT_TONEMAP_EXPOSURE_GAMMA 2 /* with the commented-out pow(ldr, 1/2.2) (toneMapper.ts:21) */
+/* A W x H float RGBA frame (gt_render's layout, pixel x + y W) as the
+ * reference's canvas shows it (src/toneMapper.ts:11-26): canvas pixel
+ * (x, y) takes pixel x + y W; RGB through the curve of `mode`; alpha 1; each
+ * channel clamped to [0, 1] and rounded to 255 steps (round half up, NaN to
+ * 0) as 8-bit RGBA.  buffers: GT_BUF_HOST (synchronous) or GT_BUF_DEVICE
+ * (asynchronous on hip_stream); rgba and out are both of that kind.  The
+ * curves are evaluated in FP64 with this library's exp/log, so host and
+ * device results are identical. */
+int gt_tonemap(const float* rgba, uint32_t width, uint32_t height, uint32_t mode, uint8_t* out, int buffers,
+               void* hip_stream);
+/* Writes 8-bit RGBA (rows top to bottom) as a PNG file: colour type 6,
+ * depth 8, filter 0, stored deflate blocks.  The reference's Save button
+ * downloads the canvas this way (src/main.ts:606-611, toDataURL('image/png')).
+ * Width and height 1..16384. */
+int gt_write_png(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
+
 const char* gt_last_error(void);
 const char* gt_version(void);
 

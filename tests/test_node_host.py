@@ -111,3 +111,28 @@ fs.writeFileSync({json.dumps(str(out))}, Buffer.concat([f0, f1].map(x => Buffer.
         want = s.render(u, 40, 30, antialiasing=True, rgba_prev=prev)["rgba"]
         np.testing.assert_array_equal(blob[frames], want)
         prev = want
+
+
+def test_tone_map_and_png_through_addon(tmp_path):
+    """toneMap / savePng (the screen pass and the Save button) equal the
+    Python binding's gt_tonemap / gt_write_png bytes."""
+    rng = np.random.default_rng(8)
+    W, H = 37, 23
+    rgba = rng.normal(0.4, 0.7, (H, W, 4)).astype(np.float32)
+    src = tmp_path / "in.bin"
+    rgba.tofile(src)
+    _node(f"""
+const rgba = new Float32Array(fs.readFileSync({json.dumps(str(src))}).buffer.slice(0));
+for (const mode of [0, 1, 2]) {{
+  fs.writeFileSync({json.dumps(str(tmp_path))} + '/m' + mode + '.bin', Buffer.from(gt.toneMap(rgba, {W}, {H}, mode)));
+}}
+gt.savePng({json.dumps(str(tmp_path / 'js.png'))}, rgba, {W}, {H}, 1);
+let err = '';
+try {{ gt.toneMap(rgba, {W} + 1, {H}); }} catch (e) {{ err = e.message; }}
+if (!err.includes('width * height * 4')) throw new Error('missing size error: ' + err);
+""", tmp_path)
+    for mode in (0, 1, 2):
+        js = np.fromfile(tmp_path / f"m{mode}.bin", np.uint8).reshape(H, W, 4)
+        np.testing.assert_array_equal(js, gt.tonemap(rgba, mode))
+    gt.write_png(tmp_path / "py.png", gt.tonemap(rgba, 1))
+    assert (tmp_path / "js.png").read_bytes() == (tmp_path / "py.png").read_bytes()

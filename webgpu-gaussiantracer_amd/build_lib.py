@@ -30,8 +30,8 @@ CXX_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
-HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip"]
-CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp"]
+HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip", "gt_image.hip"]
+CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp", "gt_png.cpp"]
 
 
 def _run(cmd, verbose):

@@ -135,12 +135,16 @@ _lib.gt_ply_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POIN
                              C.POINTER(C.c_uint32)]
 _lib.gt_free.argtypes = [_P]
 _lib.gt_free.restype = None
+_lib.gt_tonemap.argtypes = [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_int, _P]
+_lib.gt_write_png.argtypes = [C.c_char_p, _P, C.c_uint32, C.c_uint32]
+
+TONEMAP_IDENTITY, TONEMAP_EXPOSURE, TONEMAP_EXPOSURE_GAMMA = 0, 1, 2
 
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_create_raw_gpu", "gt_scene_update_device", "gt_scene_refit_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
     "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
-    "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_last_error", "gt_version",
+    "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_tonemap", "gt_write_png", "gt_last_error", "gt_version",
 ]
 
 
@@ -232,6 +236,30 @@ def read_ply(path):
     finally:
         _lib.gt_free(C.cast(p, C.c_void_p))
     return raw, deg.value
+
+
+def tonemap(rgba, mode=TONEMAP_IDENTITY):
+    """8-bit RGBA (H, W, 4) of a float frame as the reference's canvas shows it
+    (src/toneMapper.ts:11-26): the curve of mode on RGB, alpha 255."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    H, W = rgba.shape[0], rgba.shape[1]
+    assert rgba.shape == (H, W, 4)
+    out = np.empty((H, W, 4), np.uint8)
+    _check(_lib.gt_tonemap(_ptr(rgba), W, H, mode, _ptr(out), 0, None))
+    return out
+
+
+def tonemap_device(rgba_ptr, width, height, out_ptr, mode=TONEMAP_IDENTITY, stream_ptr=None):
+    """The same on device buffers (float RGBA in, 8-bit RGBA out), asynchronous on the stream."""
+    _check(_lib.gt_tonemap(rgba_ptr, width, height, mode, out_ptr, 1, stream_ptr))
+
+
+def write_png(path, rgba8):
+    """PNG file of an (H, W, 4) uint8 image (the reference's Save button, src/main.ts:606-611)."""
+    rgba8 = np.ascontiguousarray(rgba8, np.uint8)
+    H, W = rgba8.shape[0], rgba8.shape[1]
+    assert rgba8.shape == (H, W, 4)
+    _check(_lib.gt_write_png(os.fsencode(path), _ptr(rgba8), W, H))
 
 
 def debug_face_test(w2o, o, d, tmin=1e-3, tmax=1e5, margin_scale=1.0):

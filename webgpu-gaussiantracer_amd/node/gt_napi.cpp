@@ -361,6 +361,83 @@ napi_value js_render(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+bool get_u8_array(napi_env env, napi_value v, uint8_t** data, size_t* len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off = 0;
+  void* p = nullptr;
+  if (napi_get_typedarray_info(env, v, &t, len, &p, &ab, &off) != napi_ok) return false;
+  if (t != napi_uint8_array && t != napi_uint8_clamped_array) return false;
+  *data = static_cast<uint8_t*>(p);
+  return true;
+}
+
+bool get_u32(napi_env env, napi_value v, uint32_t* out) {
+  double d = 0;
+  if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0.0) || d > 4294967295.0) return false;
+  *out = (uint32_t)d;
+  return true;
+}
+
+/* toneMap(rgba: Float32Array, width, height, mode = 0) -> Uint8Array (8-bit
+ * RGBA as the reference's canvas shows the frame, src/toneMapper.ts:11-26) */
+napi_value js_tone_map(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  size_t argc = 4;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < 3) {
+    napi_throw_type_error(env, nullptr, "toneMap(rgba: Float32Array, width, height, mode?)");
+    return nullptr;
+  }
+  float* rgba = nullptr;
+  size_t len = 0;
+  uint32_t w = 0, h = 0, mode = 0;
+  if (!get_f32_array(env, argv[0], &rgba, &len) || !get_u32(env, argv[1], &w) || !get_u32(env, argv[2], &h) ||
+      (argc > 3 && !get_u32(env, argv[3], &mode))) {
+    napi_throw_type_error(env, nullptr, "toneMap(rgba: Float32Array, width, height, mode?)");
+    return nullptr;
+  }
+  if (len != (size_t)w * h * 4) {
+    napi_throw_range_error(env, nullptr, "toneMap: rgba must hold width * height * 4 floats");
+    return nullptr;
+  }
+  napi_value ab, arr;
+  void* p = nullptr;
+  NAPI_TRY(env, napi_create_arraybuffer(env, len, &p, &ab));
+  NAPI_TRY(env, napi_create_typedarray(env, napi_uint8_array, len, ab, 0, &arr));
+  const int rc = gt_tonemap(rgba, w, h, mode, static_cast<uint8_t*>(p), GT_BUF_HOST, nullptr);
+  if (rc != GT_OK) return throw_gt(env, rc);
+  return arr;
+}
+
+/* writePng(path, rgba8: Uint8Array, width, height): the Save button
+ * (src/main.ts:606-611) as a file */
+napi_value js_write_png(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  char path[4096];
+  size_t plen = 0;
+  uint8_t* px = nullptr;
+  size_t len = 0;
+  uint32_t w = 0, h = 0;
+  if (napi_get_value_string_utf8(env, argv[0], path, sizeof(path), &plen) != napi_ok ||
+      !get_u8_array(env, argv[1], &px, &len) || !get_u32(env, argv[2], &w) || !get_u32(env, argv[3], &h)) {
+    napi_throw_type_error(env, nullptr, "writePng(path: string, rgba8: Uint8Array, width, height)");
+    return nullptr;
+  }
+  if (len != (size_t)w * h * 4) {
+    napi_throw_range_error(env, nullptr, "writePng: rgba8 must hold width * height * 4 bytes");
+    return nullptr;
+  }
+  const int rc = gt_write_png(path, px, w, h);
+  if (rc != GT_OK) return throw_gt(env, rc);
+  napi_value undef;
+  napi_get_undefined(env, &undef);
+  return undef;
+}
+
 napi_value init(napi_env env, napi_value exports) {
   const napi_property_descriptor props[] = {
       {"version", nullptr, js_version, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -372,6 +449,8 @@ napi_value init(napi_env env, napi_value exports) {
       {"sceneSetAlphaMin", nullptr, js_scene_set_alpha_min, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"sceneInfo", nullptr, js_scene_info, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"render", nullptr, js_render, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"toneMap", nullptr, js_tone_map, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"writePng", nullptr, js_write_png, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
   return exports;

@@ -69,3 +69,7 @@ export declare class GaussianTracer {
 export declare function loadPly(file: string): PlyScene;
 export declare function generateScene(params: GenerateParams): Float32Array;
 export declare function version(): string;
+/** 8-bit RGBA of a rendered frame as the reference's canvas shows it (mode 0: the shader's hdrColor; 1: exposure; 2: exposure + gamma). */
+export declare function toneMap(rgba: Float32Array, width: number, height: number, mode?: 0 | 1 | 2): Uint8Array;
+/** The reference's Save button: a frame as a PNG file. */
+export declare function savePng(path: string, rgba: Float32Array, width: number, height: number, mode?: 0 | 1 | 2): void;

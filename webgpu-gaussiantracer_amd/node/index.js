@@ -90,7 +90,19 @@ class GaussianTracer {
   }
 }
 
+/* The screen pass (src/toneMapper.ts:11-26): 8-bit RGBA of a rendered frame
+ * as the canvas shows it.  mode 0 = the shader's output (hdrColor), 1 = its
+ * exposure curve 1 - exp(-1.66 hdr), 2 = that with gamma 1/2.2. */
+function toneMap(rgba, width, height, mode = 0) {
+  return addon.toneMap(rgba, width, height, mode);
+}
+
+/* The Save button (src/main.ts:606-611): a frame as a PNG file. */
+function savePng(path, rgba, width, height, mode = 0) {
+  addon.writePng(path, addon.toneMap(rgba, width, height, mode), width, height);
+}
+
 module.exports = {
-  WASDCamera, GaussianTracer, loadPly, generateScene, DEFAULTS, RAW_FLOATS: 59,
+  WASDCamera, GaussianTracer, loadPly, generateScene, toneMap, savePng, DEFAULTS, RAW_FLOATS: 59,
   version: addon.version, addon,
 };
