@@ -84,15 +84,62 @@ enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CT
 
 namespace {
 
-struct BNode {              /* inner node of the binary radix tree (64 B) */
+struct BNode {              /* inner node of the binary tree (80 B) */
   float lo[3], hi[3];
   uint32_t left, right;     /* binary refs */
   uint32_t parent;          /* BIN_NONE for the root */
   uint32_t first, count;    /* sorted proxy range */
   uint32_t leaf;            /* 1: this subtree becomes one leaf (count <= GT_LEAF_MAX) */
-  double cost;              /* SAH cost (area-weighted) */
+  float D[4];               /* least cost of the subtree as at most 1..4 wide-node slots */
+  uint8_t choice[4];        /* D[0]: 0 leaf, k wide node with k slots left; D[j>0]: 0 one slot, k split */
+  float bcost;              /* binary SAH cost with SAH-optimal leaves */
   uint32_t _pad[2];
 };
+
+/* The binary tree's leaves first: a node becomes a leaf where
+ * C_I cnt A <= C_T A + cost_L + cost_R (cnt <= 8), the SAH-optimal binary
+ * decision (bcost).  Then the least-SAH-cost 4-wide collapse over those
+ * leaves (the host builder's Collapser, GT_COLLAPSE_DP): from the children's
+ * D(., 1..4) (a single proxy costs C_I A in any number of slots), node N's
+ *   D(N, 1) = min(C_I cnt A [cnt <= 8], C_T A + min_k D(l, k) + D(r, 4 - k))
+ *   D(N, j) = min(D(N, 1), min_{0<k<j} D(l, k) + D(r, j - k)),  j = 2..4.
+ * Keeping the binary leaves matters: a collapse free to split them makes
+ * many small leaves (twice the nodes at 1M proxies) whose stack traffic the
+ * cost model does not see, and the frame got no faster. */
+__device__ __forceinline__ void dp_node(const float* dl, const float* dr, float bl, float br, uint32_t count, float A,
+                                        float* D, uint8_t* ch, uint32_t* leaf, float* bcost) {
+  const float leafc = count <= GT_LEAF_MAX ? (float)GT_SAH_ISECT * (float)count * A : INFINITY;
+  const float bsplit = 1.0f * A + bl + br;
+  if (leafc <= bsplit) { /* a binary leaf: one slot, never split */
+    *bcost = leafc;
+    D[0] = D[1] = D[2] = D[3] = leafc;
+    ch[0] = ch[1] = ch[2] = ch[3] = 0;
+    *leaf = 1u;
+    return;
+  }
+  *bcost = bsplit;
+  float dist4 = INFINITY;
+  uint8_t k4 = 1;
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) {
+    const float v = dl[k - 1] + dr[3 - k];
+    if (v < dist4) { dist4 = v; k4 = (uint8_t)k; }
+  }
+  const float wide = 1.0f * A + dist4;
+  if (leafc <= wide) { D[0] = leafc; ch[0] = 0; *leaf = 1u; }
+  else { D[0] = wide; ch[0] = k4; *leaf = 0u; }
+#pragma unroll
+  for (int j = 2; j <= 4; ++j) {
+    float best = D[0];
+    uint8_t c = 0;
+    for (int k = 1; k < j; ++k) {
+      const float v = dl[k - 1] + dr[j - k - 1];
+      if (v < best) { best = v; c = (uint8_t)k; }
+    }
+    D[j - 1] = best;
+    ch[j - 1] = c;
+  }
+}
 
 struct Front {              /* collapse work item: binary ref -> 4-wide node index */
   uint32_t bref, node4;
@@ -352,27 +399,29 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_karras(const unsigned long long
 }
 
 __device__ __forceinline__ void child_data(const BNode* bn, const float4* lbox, uint32_t ref, float* lo, float* hi,
-                                           double* cost) {
+                                           float* D, float* bcost) {
   if (ref & BIN_LEAF) {
     const uint32_t p = ref & ~BIN_LEAF;
     const float4 a = lbox[2 * (size_t)p], b = lbox[2 * (size_t)p + 1];
     lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
     hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
-    *cost = GT_SAH_ISECT * box_area(lo, hi);
+    const float c = (float)GT_SAH_ISECT * (float)box_area(lo, hi);
+    D[0] = D[1] = D[2] = D[3] = c;
+    *bcost = c;
   } else {
     const BNode& c = bn[ref];
     for (int j = 0; j < 3; ++j) { lo[j] = c.lo[j]; hi[j] = c.hi[j]; }
-    *cost = c.cost;
+    for (int j = 0; j < 4; ++j) D[j] = c.D[j];
+    *bcost = c.bcost;
   }
 }
 
-/* Completes inner node N from its two children: box union, and the
- * SAH-optimal leaf decision cost = min(C_I n A, C_T A + cost_L + cost_R). */
+/* Completes inner node N from its two children: box union and the
+ * collapse costs and choices (dp_node). */
 __device__ __forceinline__ void refit_node(BNode* __restrict__ bn, const float4* __restrict__ lbox, BNode* N) {
-  float llo[3], lhi[3], rlo[3], rhi[3];
-  double cl, cr;
-  child_data(bn, lbox, N->left, llo, lhi, &cl);
-  child_data(bn, lbox, N->right, rlo, rhi, &cr);
+  float llo[3], lhi[3], rlo[3], rhi[3], dl[4], dr[4], bl, br;
+  child_data(bn, lbox, N->left, llo, lhi, dl, &bl);
+  child_data(bn, lbox, N->right, rlo, rhi, dr, &br);
   float lo[3], hi[3];
   for (int j = 0; j < 3; ++j) {
     lo[j] = fminf(llo[j], rlo[j]);
@@ -380,12 +429,13 @@ __device__ __forceinline__ void refit_node(BNode* __restrict__ bn, const float4*
     N->lo[j] = lo[j];
     N->hi[j] = hi[j];
   }
-  const double A = box_area(lo, hi);
-  const double split = 1.0 * A + cl + cr;
-  const double leafc = GT_SAH_ISECT * (double)N->count * A;
-  const bool leaf = N->count <= GT_LEAF_MAX && leafc <= split;
-  N->cost = leaf ? leafc : split;
-  N->leaf = leaf ? 1u : 0u;
+  float D[4], bc;
+  uint8_t ch[4];
+  uint32_t leaf;
+  dp_node(dl, dr, bl, br, N->count, (float)box_area(lo, hi), D, ch, &leaf, &bc);
+  for (int j = 0; j < 4; ++j) { N->D[j] = D[j]; N->choice[j] = ch[j]; }
+  N->leaf = leaf;
+  N->bcost = bc;
 }
 
 /* Refit, phase 1: one thread per leaf climbs while it is the second child to
@@ -516,23 +566,25 @@ __device__ void collapse_one(const Front e, Front* __restrict__ out, unsigned in
     ch[0] = e.bref;
     m = 1;
   } else {
-    ch[0] = bn[e.bref].left;
-    ch[1] = bn[e.bref].right;
-    m = 2;
-    while (m < 4) {
-      int best = -1;
-      double best_area = -1.0;
-      for (int c = 0; c < m; ++c) {
-        if (!openable(bn, ch[c])) continue;
-        const double a = box_area(bn[ch[c]].lo, bn[ch[c]].hi);
-        if (a > best_area) { best_area = a; best = c; }
+    /* the slots chosen by the collapse DP, left to right: a stack of
+     * (subtree, slots) pairs whose slot counts sum to 4 */
+    const BNode& R = bn[e.bref];
+    uint32_t sref[4];
+    int sj[4], sp = 0;
+    sref[sp] = R.right; sj[sp++] = 4 - R.choice[0];
+    sref[sp] = R.left; sj[sp++] = R.choice[0];
+    m = 0;
+    while (sp > 0) {
+      --sp;
+      const uint32_t r = sref[sp];
+      const int j = sj[sp];
+      const int c = (j == 1 || (r & BIN_LEAF)) ? 0 : bn[r].choice[j - 1];
+      if (c == 0 || sp + 2 > 4) {
+        ch[m++] = r;
+        continue;
       }
-      if (best < 0) break;
-      const uint32_t o = ch[best];
-      for (int c = m; c > best + 1; --c) ch[c] = ch[c - 1];
-      ch[best] = bn[o].left;
-      ch[best + 1] = bn[o].right;
-      ++m;
+      sref[sp] = bn[r].right; sj[sp++] = j - c;
+      sref[sp] = bn[r].left; sj[sp++] = c;
     }
   }
   gt_node4 nd;
@@ -637,11 +689,13 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
                                                  const uint32_t* __restrict__ items, unsigned int* ctl,
                                                  uint32_t* __restrict__ perm, uint32_t bn_cap) {
   __shared__ float s_lo[2][3][PLOC_N], s_hi[2][3][PLOC_N];
-  __shared__ double s_cost[2][PLOC_N];
+  __shared__ float s_D[2][4][PLOC_N]; /* collapse costs of the clusters (dp_node) */
+  __shared__ float s_bc[2][PLOC_N];   /* binary costs of the clusters */
   __shared__ uint32_t s_ref[2][PLOC_N], s_cnt[2][PLOC_N];
   __shared__ uint32_t s_nn[PLOC_N];
   __shared__ float s_blo[3][PLOC_N], s_bhi[3][PLOC_N]; /* made nodes, by local slot */
-  __shared__ double s_bcost[PLOC_N];
+  __shared__ float s_bD[4][PLOC_N], s_bbc[PLOC_N];
+  __shared__ uint32_t s_bch[PLOC_N]; /* the four choices, one byte each */
   __shared__ uint32_t s_bl[PLOC_N], s_br[PLOC_N], s_bcnt[PLOC_N], s_bleaf[PLOC_N], s_bfirst[PLOC_N];
   __shared__ uint32_t s_itend[PLOC_N], s_leafpos[PLOC_N];
   __shared__ uint32_t s_wsum[PLOC_N / 64];
@@ -666,7 +720,10 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
       s_lo[0][0][tid] = a.x; s_lo[0][1][tid] = a.y; s_lo[0][2][tid] = a.z;
       s_hi[0][0][tid] = b.x; s_hi[0][1][tid] = b.y; s_hi[0][2][tid] = b.z;
       const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
-      s_cost[0][tid] = GT_SAH_ISECT * box_area(lo, hi);
+      const float c = (float)GT_SAH_ISECT * (float)box_area(lo, hi);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s_D[0][q][tid] = c;
+      s_bc[0][tid] = c;
       s_ref[0][tid] = PLOC_NODE_LEAF | tid;
       s_cnt[0][tid] = 1u;
     }
@@ -721,24 +778,28 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
           hi[q] = fmaxf(s_hi[cur][q][tid], s_hi[cur][q][j]);
         }
         const uint32_t cnt = s_cnt[cur][tid] + s_cnt[cur][j];
-        const double A = box_area(lo, hi);
-        const double split = 1.0 * A + s_cost[cur][tid] + s_cost[cur][j];
-        const double leafc = GT_SAH_ISECT * (double)cnt * A;
-        const bool leaf = cnt <= GT_LEAF_MAX && leafc <= split;
-        const double cost = leaf ? leafc : split;
+        float dl[4], dr[4], D[4], bc;
+        uint8_t chs[4];
+        uint32_t leaf;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { dl[q] = s_D[cur][q][tid]; dr[q] = s_D[cur][q][j]; }
+        dp_node(dl, dr, s_bc[cur][tid], s_bc[cur][j], cnt, (float)box_area(lo, hi), D, chs, &leaf, &bc);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           s_blo[q][l] = lo[q]; s_bhi[q][l] = hi[q];
           s_lo[nxt][q][pos] = lo[q]; s_hi[nxt][q][pos] = hi[q];
         }
-        s_bcost[l] = cost;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { s_bD[q][l] = D[q]; s_D[nxt][q][pos] = D[q]; }
+        s_bbc[l] = bc;
+        s_bc[nxt][pos] = bc;
+        s_bch[l] = (uint32_t)chs[0] | ((uint32_t)chs[1] << 8) | ((uint32_t)chs[2] << 16) | ((uint32_t)chs[3] << 24);
         s_bl[l] = s_ref[cur][tid];
         s_br[l] = s_ref[cur][j];
         s_bcnt[l] = cnt;
-        s_bleaf[l] = leaf ? 1u : 0u;
+        s_bleaf[l] = leaf;
         s_ref[nxt][pos] = l;
         s_cnt[nxt][pos] = cnt;
-        s_cost[nxt][pos] = cost;
       } else if (keep) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -747,7 +808,9 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
         }
         s_ref[nxt][pos] = s_ref[cur][tid];
         s_cnt[nxt][pos] = s_cnt[cur][tid];
-        s_cost[nxt][pos] = s_cost[cur][tid];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_D[nxt][q][pos] = s_D[cur][q][tid];
+        s_bc[nxt][pos] = s_bc[cur][tid];
       }
       __syncthreads();
       if (total >= n) { /* no merge: only with NaN boxes */
@@ -792,7 +855,12 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
       B->first = first + s_bfirst[l];
       B->count = s_bcnt[l];
       B->leaf = s_bleaf[l];
-      B->cost = s_bcost[l];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        B->D[q] = s_bD[q][l];
+        B->choice[q] = (uint8_t)(s_bch[l] >> (8 * q));
+      }
+      B->bcost = s_bbc[l];
     }
     if (tid < count) {
       const uint32_t lp = s_leafpos[tid];
@@ -982,9 +1050,10 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
     gt_set_error("device build: %u proxies for %u Gaussians (limit %u)", np, n, GT_MAX_PROXIES);
     return GT_E_LIMIT;
   }
-  /* a 4-wide node has >= 2 children (the root aside) and >= 4 unless all its
-   * children are leaves, so nodes <= 2 np / 3 + 1 */
-  const uint32_t cap = np ? (uint32_t)((2ull * np) / 3ull + 2ull) : 1u;
+  /* every wide node but a one-leaf root has >= 2 children (the collapse DP
+   * splits a node's slots between both of its binary children), and the
+   * leaves hold >= 1 proxy each, so nodes <= np - 1 (1 for np <= 1) */
+  const uint32_t cap = np > 1u ? np : 1u;
   GT_TRY(gt_devbuf_reserve(s->prox, (size_t)np * 64));
   GT_TRY(gt_devbuf_reserve(s->nodes, (size_t)cap * sizeof(gt_node4)));
   uint32_t root = GT_NODE_EMPTY, n_nodes = 0, need = 0, depth = 0;
