@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # this script scaled by the counters' reading for a 1 GiB device copy
 # (x2.000 and x1.000): profiles/k_forward_1M_1080p_k16.md.  Reported as
 # roofline.traffic for that workload only.
-TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 565762010, "write_bytes": 172084122}
+TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 543963750, "write_bytes": 171515637}
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
 
 
