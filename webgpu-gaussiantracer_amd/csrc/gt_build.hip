@@ -108,7 +108,7 @@ struct BNode {              /* inner node of the binary tree (80 B) */
  * cost model does not see, and the frame got no faster. */
 __device__ __forceinline__ void dp_node(const float* dl, const float* dr, float bl, float br, uint32_t count, float A,
                                         float* D, uint8_t* ch, uint32_t* leaf, float* bcost) {
-  const float leafc = count <= GT_LEAF_MAX ? (float)GT_SAH_ISECT * (float)count * A : INFINITY;
+  const float leafc = count <= GT_LEAF_MAX ? ((float)GT_SAH_ISECT * (float)count + (float)GT_SAH_LEAF) * A : INFINITY;
   const float bsplit = 1.0f * A + bl + br;
   if (leafc <= bsplit) { /* a binary leaf: one slot, never split */
     *bcost = leafc;
@@ -405,7 +405,7 @@ __device__ __forceinline__ void child_data(const BNode* bn, const float4* lbox, 
     const float4 a = lbox[2 * (size_t)p], b = lbox[2 * (size_t)p + 1];
     lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
     hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
-    const float c = (float)GT_SAH_ISECT * (float)box_area(lo, hi);
+    const float c = ((float)GT_SAH_ISECT + (float)GT_SAH_LEAF) * (float)box_area(lo, hi);
     D[0] = D[1] = D[2] = D[3] = c;
     *bcost = c;
   } else {
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
       s_lo[0][0][tid] = a.x; s_lo[0][1][tid] = a.y; s_lo[0][2][tid] = a.z;
       s_hi[0][0][tid] = b.x; s_hi[0][1][tid] = b.y; s_hi[0][2][tid] = b.z;
       const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
-      const float c = (float)GT_SAH_ISECT * (float)box_area(lo, hi);
+      const float c = ((float)GT_SAH_ISECT + (float)GT_SAH_LEAF) * (float)box_area(lo, hi);
 #pragma unroll
       for (int q = 0; q < 4; ++q) s_D[0][q][tid] = c;
       s_bc[0][tid] = c;

@@ -39,6 +39,9 @@
 #ifndef GT_SAH_ISECT
 #define GT_SAH_ISECT 0.5
 #endif
+#ifndef GT_SAH_LEAF
+#define GT_SAH_LEAF 0.0 /* SAH cost of a leaf besides its proxy tests (C_L A), both builders */
+#endif
 
 static inline uint32_t gt_leaf_ref(uint32_t first, uint32_t count) {
   return GT_LEAF_BIT | ((count - 1u) << GT_LEAF_COUNT_SHIFT) | first;

@@ -137,6 +137,7 @@ struct Builder {
     work.push_back({0, 0, n, 0});
     constexpr int NB = 32;
     const double C_TRAV = 1.0, C_ISECT = GT_SAH_ISECT;
+    const double lam = GT_SAH_LEAF / GT_SAH_ISECT; /* leaf cost in proxy tests */
     while (!work.empty()) {
       Task t = work.back();
       work.pop_back();
@@ -193,14 +194,14 @@ struct Builder {
             acc.grow(bin_box[b]);
             cnt += bin_cnt[b];
             if (cnt == 0 || right_cnt[b + 1] == 0) continue;
-            const double c = acc.area() * cnt + right_area[b + 1] * right_cnt[b + 1];
+            const double c = acc.area() * (cnt + lam) + right_area[b + 1] * (right_cnt[b + 1] + lam);
             if (c < best_cost) { best_cost = c; best_axis = ax; best_bin = b; }
           }
         }
       }
       const double parent_area = std::max(bb.area(), 1e-30);
       const double split_cost = C_TRAV + C_ISECT * best_cost / parent_area;
-      const double leaf_cost = C_ISECT * t.count;
+      const double leaf_cost = C_ISECT * (t.count + lam);
       if (t.count <= GT_LEAF_MAX && (best_axis < 0 || leaf_cost <= split_cost)) { make_leaf(); continue; }
       uint32_t mid;
       if (best_axis >= 0) {
@@ -286,7 +287,7 @@ struct Collapser {
     const B2Node& b = b2[n];
     const double A = b.box.area();
     if (b.left < 0) {
-      for (int j = 1; j <= 4; ++j) { D[n][j] = GT_SAH_ISECT * (double)b.count * A; choice[n][j] = 0; }
+      for (int j = 1; j <= 4; ++j) { D[n][j] = (GT_SAH_ISECT * (double)b.count + GT_SAH_LEAF) * A; choice[n][j] = 0; }
       return;
     }
     dp(b.left);
@@ -298,7 +299,7 @@ struct Collapser {
       if (v < dist4) { dist4 = v; k4 = k; }
     }
     const double wide = 1.0 * A + dist4;
-    const double leafc = count[n] <= GT_LEAF_MAX ? GT_SAH_ISECT * (double)count[n] * A : INFINITY;
+    const double leafc = count[n] <= GT_LEAF_MAX ? (GT_SAH_ISECT * (double)count[n] + GT_SAH_LEAF) * A : INFINITY;
     if (leafc <= wide) { D[n][1] = leafc; choice[n][1] = 0; }
     else { D[n][1] = wide; choice[n][1] = k4; }
     for (int j = 2; j <= 4; ++j) {
