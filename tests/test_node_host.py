@@ -136,3 +136,26 @@ if (!err.includes('width * height * 4')) throw new Error('missing size error: ' 
         np.testing.assert_array_equal(js, gt.tonemap(rgba, mode))
     gt.write_png(tmp_path / "py.png", gt.tonemap(rgba, 1))
     assert (tmp_path / "js.png").read_bytes() == (tmp_path / "py.png").read_bytes()
+
+
+@pytest.mark.gpu
+def test_render_into_reused_array(tmp_path):
+    """opts.out: the frame is written into (and returned as) the caller's
+    Float32Array; it equals a render into a new array bit for bit."""
+    _node("""
+const raw = gt.generateScene({n: 3000, nClusters: 2, boxHalf: 2, seed: 5});
+const tr = new gt.GaussianTracer(raw, 3000, 0.1);
+const cam = new gt.WASDCamera({position: [0, 0, 8], target: [0, 0, 0]});
+const opts = {width: 96, height: 64, alpha_min: 0.1, hit_array_size: 6, T_min: 0.03};
+const a = tr.render(cam, opts);
+const buf = new Float32Array(96 * 64 * 4).fill(7);
+for (let i = 0; i < 2; ++i) {
+  const b = tr.render(cam, Object.assign({out: buf}, opts));
+  if (b !== buf) throw new Error('out not returned');
+}
+for (let i = 0; i < a.length; ++i) if (a[i] !== buf[i]) throw new Error('differs at ' + i);
+let err = '';
+try { tr.render(cam, Object.assign({out: new Float32Array(5)}, opts)); } catch (e) { err = e.message; }
+if (!err.includes('opts.out')) throw new Error('missing size error: ' + err);
+tr.destroy();
+""", tmp_path)

@@ -892,7 +892,7 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
   struct Tmp {
     void* p = nullptr;
     ~Tmp() { if (p) (void)hipFree(p); }
-  } t_rgba, t_hits, t_acc, t_rounds, t_T, t_stats, t_cids, t_ncomp;
+  } t_hits, t_acc, t_rounds, t_T, t_stats, t_cids, t_ncomp;
   auto alloc = [&](Tmp& t, size_t bytes) -> int {
     hipError_t e = hipMalloc(&t.p, std::max<size_t>(bytes, 4));
     if (e != hipSuccess) {
@@ -902,8 +902,10 @@ extern "C" int gt_render(gt_scene* s, const gt_uniforms* u, uint32_t width, uint
     }
     return GT_OK;
   };
-  if ((rc = alloc(t_rgba, R * 16)) != GT_OK) return rc;
-  f.rgba = (float*)t_rgba.p;
+  /* the image goes through a buffer the scene keeps: no allocation per
+   * frame once the largest frame size has been seen */
+  if ((rc = gt_devbuf_reserve(s->himg, R * 16)) != GT_OK) return rc;
+  f.rgba = (float*)s->himg.p;
   if (out->hit_ids) {
     if ((rc = alloc(t_hits, R * MR * u->k * 4)) != GT_OK) return rc;
     f.hit_ids = (uint32_t*)t_hits.p;

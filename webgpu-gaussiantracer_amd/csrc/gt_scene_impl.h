@@ -49,6 +49,7 @@ struct gt_scene {
   DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
   DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
   DevBuf aa;  /* antialiasing: the new frame before it is averaged in */
+  DevBuf himg; /* gt_render with host buffers: the frame on the device */
   uint32_t rc_cap = 0; /* checkpoint capacity the last recompute needed */
   gt_tuning tune{};
   gt_dev_scene dev{};

@@ -373,15 +373,20 @@ class Scene:
         return dict(packed=packed, proxy_ids=ids, o2w=o2w, w2o=w2o)
 
     def render(self, u, width, height, max_rounds=0, want_T=False, want_stats=False, checkpoint_cap=0,
-               antialiasing=False, rgba_prev=None):
+               antialiasing=False, rgba_prev=None, out=None):
         """traceRays(W, H) with host buffers: returns rgba (H, W, 4) and, on request,
         hit lists (H*W, max_rounds, k), accept masks, round counts, T, stats and
         the backward checkpoint (composited ids per ray, capacity checkpoint_cap;
         checkpoint_cap=-1 returns only the per-ray counts).  antialiasing: the
         reference's jittered-ray path; with u.accumulated_frames > 0 the frame
-        is averaged into rgba_prev (the previous average)."""
+        is averaged into rgba_prev (the previous average).  out: a float32
+        (height, width, 4) array to render into (a render loop reuses it)."""
         R = width * height
-        rgba = np.zeros((height, width, 4), np.float32)
+        if out is not None:
+            assert out.dtype == np.float32 and out.shape == (height, width, 4) and out.flags.c_contiguous
+            rgba = out
+        else:
+            rgba = np.zeros((height, width, 4), np.float32)
         if rgba_prev is not None:
             rgba[...] = np.asarray(rgba_prev, np.float32).reshape(height, width, 4)
         out = {"rgba": rgba}

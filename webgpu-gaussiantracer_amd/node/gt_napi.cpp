@@ -334,11 +334,30 @@ napi_value js_render(napi_env env, napi_callback_info info) {
    * opts.prev, the previous average */
   const bool aa = bool_prop(env, o, "antialiasing", false);
   u.accumulated_frames = (uint32_t)num_prop(env, o, "accumulatedFrames", 0);
+  /* opts.out: a Float32Array(W*H*4) to render into and return (a render
+   * loop reuses it); otherwise a new array */
   float* rgba = nullptr;
-  napi_value arr = new_f32_array(env, (size_t)W * H * 4, &rgba);
+  napi_value arr = nullptr;
+  bool has_out = false;
+  if (napi_has_named_property(env, o, "out", &has_out) == napi_ok && has_out) {
+    napi_value ov;
+    size_t olen = 0;
+    napi_valuetype ty = napi_undefined;
+    if (napi_get_named_property(env, o, "out", &ov) == napi_ok) napi_typeof(env, ov, &ty);
+    if (ty != napi_undefined && ty != napi_null) {
+      if (!get_f32_array(env, ov, &rgba, &olen) || olen != (size_t)W * H * 4) {
+        napi_throw_type_error(env, nullptr, "opts.out must be a Float32Array of width * height * 4");
+        return nullptr;
+      }
+      arr = ov;
+    }
+  }
   if (!arr) {
-    napi_throw_error(env, nullptr, "allocation failed");
-    return nullptr;
+    arr = new_f32_array(env, (size_t)W * H * 4, &rgba);
+    if (!arr) {
+      napi_throw_error(env, nullptr, "allocation failed");
+      return nullptr;
+    }
   }
   if (aa && u.accumulated_frames) {
     bool has = false;
@@ -351,7 +370,7 @@ napi_value js_render(napi_env env, napi_callback_info info) {
       napi_throw_type_error(env, nullptr, "antialiasing with accumulatedFrames > 0 needs opts.prev: Float32Array(W*H*4)");
       return nullptr;
     }
-    std::memcpy(rgba, prev, (size_t)W * H * 4 * sizeof(float));
+    if (prev != rgba) std::memmove(rgba, prev, (size_t)W * H * 4 * sizeof(float));
   }
   gt_frame f{};
   f.rgba = rgba;

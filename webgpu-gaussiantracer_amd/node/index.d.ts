@@ -17,6 +17,7 @@ export interface RenderOptions {
   antialiasing?: boolean;   // the reference's compiled-out AA path: jittered rays (default false)
   accumulatedFrames?: number; // frames already averaged (the uniform of main.ts:455, 597)
   prev?: Float32Array;      // the previous average, required when accumulatedFrames > 0
+  out?: Float32Array;       // width*height*4 floats to render into and return (reused by a render loop)
 }
 
 export declare const DEFAULTS: Readonly<Required<Omit<RenderOptions, 'width' | 'height'>>>;
