@@ -100,8 +100,10 @@ int gt_scene_set_alpha_min(gt_scene* s, double alpha_min);
  * (webrtx/src/wasm_bvh_builder.ts:260-377, webrtx/bvh/src/lib.rs:253-347):
  * activations, records, proxy transforms and boxes are computed by the same
  * code as gt_scene_create_raw (bit-identical); the BVH is a GPU radix tree
- * over Morton codes with SAH-chosen leaves, collapsed to the same 4-wide
- * node format.  Hit lists and pixels equal those of the host-built scene.
+ * over Morton codes whose subtrees of up to 256 leaves are rebuilt by
+ * locally-ordered clustering, with SAH-chosen leaves and the least-SAH-cost
+ * collapse to the same 4-wide node format.  Hit lists and pixels equal
+ * those of the host-built scene.
  * raw_dev is copied into the scene; the call returns when the build is done
  * (it synchronises hip_stream).  gt_scene_set_alpha_min rebuilds on the GPU. */
 int gt_scene_create_device(const float* raw_dev, uint32_t n, double alpha_min, void* hip_stream, gt_scene** out);
@@ -123,7 +125,7 @@ int gt_scene_update_device(gt_scene* s, const float* raw_dev, void* hip_stream);
  * are recomputed, and the node boxes are refit level by level.  Which hits a
  * ray finds does not depend on the tree, so renders equal those of a fresh
  * build of the same parameters, bit for bit.  A host-built tree keeps its
- * SAH quality, which a rebuild on the GPU (Morton order) would not.  If the
+ * binned-SAH quality, and a refit costs less than a rebuild.  If the
  * set of Gaussians with a proxy changed, the scene is rebuilt on the GPU
  * instead and *rebuilt (if not NULL) is set to 1, else 0.  The scene then
  * holds device data only (gt_scene_get_host_data downloads it).  Returns

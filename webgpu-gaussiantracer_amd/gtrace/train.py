@@ -15,8 +15,8 @@ loop the library's pieces make possible, with the parameters resident in HBM:
 Which hits a ray finds does not depend on the tree, so both updates give the
 same renders and gradients bit for bit.  They differ in speed: the refit is
 cheaper than a rebuild and keeps the tree of the first build, by default the
-host's binned-SAH tree, which renders faster than the GPU's Morton-order
-tree (by 20 % on a dense 200k-Gaussian cluster).
+host's binned-SAH tree, which renders up to 2 % faster than the GPU's
+tree (a Morton radix tree with clustered subtrees).
 
 Default learning rates are those of the 3D Gaussian Splatting reference
 optimiser (Kerbl et al. 2023): means 1.6e-4 x spatial scale, f_dc 2.5e-3,
@@ -63,7 +63,7 @@ class Trainer:
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         sp = self.stream.cuda_stream
         if build not in ("host", "gpu"):
-            raise ValueError("build must be 'host' (binned SAH) or 'gpu' (Morton radix tree)")
+            raise ValueError("build must be 'host' (binned SAH) or 'gpu' (radix tree with PLOC subtrees)")
         if build == "host" and not refit:
             raise ValueError("a host-built tree is kept by refit; rebuilds (refit=False) need build='gpu'")
         self.refit = bool(refit)
