@@ -119,3 +119,33 @@ def test_tuning_never_changes_results():
         for key in ("hit_ids", "accept", "n_rounds"):
             assert np.array_equal(r[key], ref[key]), (t, key)
         assert r["rgba"].tobytes() == ref["rgba"].tobytes(), t
+
+
+@pytest.mark.parametrize("k,alpha", [(4, 0.1), (32, 0.1), (16, 0.01)])
+def test_batch_k_and_alpha_of_the_batched_workload(k, alpha):
+    """The batched workload's settings: alpha_min in {0.01, 0.1} (scene filter
+    and uniform) and k from {4, 8, 16, 32}; a batch shares one k."""
+    print(f"case batch k={k} alpha={alpha}", flush=True)
+    raw = gt.generate_raw(n=60000, seed=21, config="c2", n_clusters=6)
+    s = gt.Scene(raw, alpha_min=alpha)
+    c = raw[0::6, :3].mean(axis=0)
+    rng = np.random.default_rng(7)
+    us = []
+    for w, h in SIZES:
+        off = rng.normal(size=3)
+        off = off / np.linalg.norm(off) * 2.5
+        us.append(uniforms(list(c + off), list(c), w, h, hfov_deg=55, alpha_min=alpha, k=k, T_min=1e-3))
+    single = _single(s, us, 48)
+    _same(_render_batch(s, us, 48)[0], single)
+
+
+def test_batch_rejects_mixed_k():
+    s, c = _scene()
+    us = _cams(c, k=8)
+    us[1] = uniforms(list(c + 2.5), list(c), SIZES[1][0], SIZES[1][1], hfov_deg=55, alpha_min=0.01, k=16,
+                     T_min=1e-3)
+    dev = torch.device("cuda:0")
+    imgs = [torch.zeros((h, w, 4), device=dev) for (w, h) in SIZES]
+    frames = [dict(rgba=t.data_ptr()) for t in imgs]
+    with pytest.raises(gt.GTError, match="shares k"):
+        gt.Batch(s, us, SIZES, frames)
