@@ -16,12 +16,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
-#include <array>
+#include <thread>
 #include <vector>
 
 #include "../../include/gt.h"
@@ -40,10 +42,26 @@
   } while (0)
 
 /* ---------------- activations (src/GS_ply.ts:187-196, 225-235) ---------------- */
+/* worker threads for host stages over n items (1 for small inputs) */
+static unsigned host_threads(size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (unsigned)std::min<size_t>(std::min(hw, 16u), std::max<size_t>(1, n / 16384));
+}
+
+/* f(begin, end, chunk) over [0, n) in `chunks` contiguous chunks on threads */
+template <class F>
+static void parallel_chunks(size_t n, unsigned chunks, F f) {
+  if (chunks <= 1) { f((size_t)0, n, 0u); return; }
+  std::vector<std::thread> th;
+  for (unsigned c = 0; c < chunks; ++c) th.emplace_back([=, &f]() { f(n * c / chunks, n * (c + 1) / chunks, c); });
+  for (auto& x : th) x.join();
+}
+
 static void activate_raw(HostScene& h) {
   h.act.assign((size_t)h.n * 8, 0.0);
   h.packed.assign((size_t)h.n * 80, 0.0f);
-  for (uint32_t i = 0; i < h.n; ++i) {
+  parallel_chunks(h.n, host_threads(h.n), [&](size_t i0, size_t i1, unsigned) {
+  for (size_t i = i0; i < i1; ++i) {
     float g[12], sh[48];
     gt_prep_activate(&h.raw[(size_t)i * GT_RAW_FLOATS], &h.act[(size_t)i * 8], g, sh);
     float* o = &h.packed[(size_t)i * 80];
@@ -54,6 +72,7 @@ static void activate_raw(HostScene& h) {
     for (int j = 0; j < 16; ++j)
       for (int c = 0; c < 3; ++c) o[16 + 4 * j + c] = sh[3 * j + c];
   }
+  });
 }
 
 static void activate_packed(HostScene& h) {
@@ -69,20 +88,31 @@ static void activate_packed(HostScene& h) {
 
 /* ---------------- proxies (src/main.ts:73-94, webrtx/bvh/src/lib.rs:501-505) ---------------- */
 static void build_proxies(HostScene& h, double alpha_min) {
+  const unsigned chunks = host_threads(h.n);
+  std::vector<std::vector<uint32_t>> ids(chunks);
+  std::vector<std::vector<float>> o2w(chunks), w2o(chunks);
+  parallel_chunks(h.n, chunks, [&](size_t i0, size_t i1, unsigned c) {
+    for (size_t i = i0; i < i1; ++i) {
+      float m[12], inv[12];
+      if (!gt_prep_proxy(&h.act[i * 8], &h.packed[i * 80], alpha_min, m, inv)) continue;
+      ids[c].push_back((uint32_t)i);
+      o2w[c].insert(o2w[c].end(), m, m + 12);
+      w2o[c].insert(w2o[c].end(), inv, inv + 12);
+    }
+  });
   h.proxy_ids.clear();
   h.o2w.clear();
   h.w2o.clear();
-  for (uint32_t i = 0; i < h.n; ++i) {
-    float m[12], inv[12];
-    if (!gt_prep_proxy(&h.act[(size_t)i * 8], &h.packed[(size_t)i * 80], alpha_min, m, inv)) continue;
-    h.proxy_ids.push_back(i);
-    h.o2w.insert(h.o2w.end(), m, m + 12);
-    h.w2o.insert(h.w2o.end(), inv, inv + 12);
+  for (unsigned c = 0; c < chunks; ++c) { /* in Gaussian order, as one pass would make them */
+    h.proxy_ids.insert(h.proxy_ids.end(), ids[c].begin(), ids[c].end());
+    h.o2w.insert(h.o2w.end(), o2w[c].begin(), o2w[c].end());
+    h.w2o.insert(h.w2o.end(), w2o[c].begin(), w2o[c].end());
   }
 }
 
 /* ---------------- BVH ---------------- */
 namespace {
+
 
 struct Box {
   float lo[3], hi[3];
@@ -128,115 +158,217 @@ struct Builder {
     }
   }
 
+  struct Task { int32_t node; uint32_t first, count, depth; };
+
+  /* One node of the binned-SAH build: its box, then a leaf or a split whose
+   * two children are appended to ns and pushed on work.  It reads only the
+   * node's own range of order, so subtrees may be built concurrently. */
+  static constexpr int NB = 32;
+
+  /* Box and centroid bounds of a range, and its proxies binned by centroid
+   * on the three axes (axes with no extent are left empty).  Large ranges
+   * run in chunks on threads; min, max and counts are exact, so the result
+   * does not depend on the chunking. */
+  void bounds(uint32_t first, uint32_t count, bool spawn, Box& bb, Box& cb) const {
+    const unsigned nth = spawn && count >= (1u << 16) ? host_threads(count) : 1u;
+    std::vector<Box> pb(nth), pc(nth);
+    parallel_chunks(count, nth, [&](size_t a, size_t b, unsigned c) {
+      Box x, y;
+      x.empty();
+      y.empty();
+      for (size_t i = first + a; i < first + b; ++i) {
+        const uint32_t p = order[i];
+        x.grow(boxes[p]);
+        Box q;
+        for (int j = 0; j < 3; ++j) q.lo[j] = q.hi[j] = cent[3 * p + j];
+        y.grow(q);
+      }
+      pb[c] = x;
+      pc[c] = y;
+    });
+    bb.empty();
+    cb.empty();
+    for (unsigned c = 0; c < nth; ++c) { bb.grow(pb[c]); cb.grow(pc[c]); }
+  }
+
+  void bin(uint32_t first, uint32_t count, bool spawn, const Box& cb, Box (&bin_box)[3][NB],
+           uint32_t (&bin_cnt)[3][NB]) const {
+    const unsigned nth = spawn && count >= (1u << 16) ? host_threads(count) : 1u;
+    struct Bins { Box box[3][NB]; uint32_t cnt[3][NB]; };
+    std::vector<Bins> part(nth);
+    float scale[3];
+    bool use[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const float ext = cb.hi[ax] - cb.lo[ax];
+      use[ax] = ext > 0.0f;
+      scale[ax] = use[ax] ? NB / ext : 0.0f;
+    }
+    parallel_chunks(count, nth, [&](size_t a, size_t b, unsigned c) {
+      Bins& B = part[c];
+      for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k < NB; ++k) { B.box[ax][k].empty(); B.cnt[ax][k] = 0; }
+      for (size_t i = first + a; i < first + b; ++i) {
+        const uint32_t p = order[i];
+        for (int ax = 0; ax < 3; ++ax) {
+          if (!use[ax]) continue;
+          int k = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale[ax]);
+          k = std::min(std::max(k, 0), NB - 1);
+          B.cnt[ax][k]++;
+          B.box[ax][k].grow(boxes[p]);
+        }
+      }
+    });
+    for (int ax = 0; ax < 3; ++ax)
+      for (int k = 0; k < NB; ++k) {
+        bin_box[ax][k].empty();
+        bin_cnt[ax][k] = 0;
+        for (unsigned c = 0; c < nth; ++c) {
+          bin_box[ax][k].grow(part[c].box[ax][k]);
+          bin_cnt[ax][k] += part[c].cnt[ax][k];
+        }
+      }
+  }
+
+  void step(const Task& t, std::vector<B2Node>& ns, std::vector<Task>& work, uint32_t& maxd, bool spawn) {
+    const double C_TRAV = 1.0, C_ISECT = GT_SAH_ISECT;
+    const double lam = GT_SAH_LEAF / GT_SAH_ISECT; /* leaf cost in proxy tests */
+    maxd = std::max(maxd, t.depth);
+    Box bb, cb;
+    bounds(t.first, t.count, spawn, bb, cb);
+    ns[t.node].box = bb;
+    auto make_leaf = [&]() {
+      ns[t.node].left = -1;
+      ns[t.node].first = t.first;
+      ns[t.node].count = t.count;
+    };
+    if (t.count <= 1) { make_leaf(); return; }
+    // best binned SAH split over the three axes
+    int best_axis = -1, best_bin = -1;
+    double best_cost = INFINITY;
+    if (t.depth < 48) {
+      Box bins[3][NB];
+      uint32_t counts[3][NB];
+      bin(t.first, t.count, spawn, cb, bins, counts);
+      for (int ax = 0; ax < 3; ++ax) {
+        const float ext = cb.hi[ax] - cb.lo[ax];
+        if (!(ext > 0.0f)) continue;
+        const Box* bin_box = bins[ax];
+        const uint32_t* bin_cnt = counts[ax];
+        double right_area[NB];
+        uint32_t right_cnt[NB];
+        Box acc;
+        acc.empty();
+        uint32_t cnt = 0;
+        for (int b = NB - 1; b > 0; --b) {
+          acc.grow(bin_box[b]);
+          cnt += bin_cnt[b];
+          right_area[b] = acc.area();
+          right_cnt[b] = cnt;
+        }
+        acc.empty();
+        cnt = 0;
+        for (int b = 0; b < NB - 1; ++b) {
+          acc.grow(bin_box[b]);
+          cnt += bin_cnt[b];
+          if (cnt == 0 || right_cnt[b + 1] == 0) continue;
+          const double c = acc.area() * (cnt + lam) + right_area[b + 1] * (right_cnt[b + 1] + lam);
+          if (c < best_cost) { best_cost = c; best_axis = ax; best_bin = b; }
+        }
+      }
+    }
+    const double parent_area = std::max(bb.area(), 1e-30);
+    const double split_cost = C_TRAV + C_ISECT * best_cost / parent_area;
+    const double leaf_cost = C_ISECT * (t.count + lam);
+    if (t.count <= GT_LEAF_MAX && (best_axis < 0 || leaf_cost <= split_cost)) { make_leaf(); return; }
+    uint32_t mid;
+    if (best_axis >= 0) {
+      const int ax = best_axis;
+      const float ext = cb.hi[ax] - cb.lo[ax];
+      const float scale = NB / ext;
+      auto it = std::partition(order.begin() + t.first, order.begin() + t.first + t.count, [&](uint32_t p) {
+        int b = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale);
+        b = std::min(std::max(b, 0), NB - 1);
+        return b <= best_bin;
+      });
+      mid = (uint32_t)(it - order.begin());
+    } else {
+      mid = t.first + t.count;  // force the median fallback below
+    }
+    if (mid == t.first || mid == t.first + t.count) {
+      // degenerate or too deep: object median on the widest centroid axis
+      int ax = 0;
+      for (int j = 1; j < 3; ++j)
+        if (cb.hi[j] - cb.lo[j] > cb.hi[ax] - cb.lo[ax]) ax = j;
+      mid = t.first + t.count / 2;
+      std::nth_element(order.begin() + t.first, order.begin() + mid, order.begin() + t.first + t.count,
+                       [&](uint32_t a, uint32_t b) {
+                         if (cent[3 * a + ax] != cent[3 * b + ax]) return cent[3 * a + ax] < cent[3 * b + ax];
+                         return a < b;
+                       });
+    }
+    const int32_t l = (int32_t)ns.size();
+    ns.emplace_back();
+    const int32_t r = (int32_t)ns.size();
+    ns.emplace_back();
+    ns[t.node].left = l;
+    ns[t.node].right = r;
+    work.push_back({r, mid, t.first + t.count - mid, t.depth + 1});
+    work.push_back({l, t.first, mid - t.first, t.depth + 1});
+  }
+
+  /* The top levels run here; subtrees of at most `grain` proxies go to
+   * worker threads, each into a node list of its own, merged in a fixed
+   * order afterwards.  A node's split depends only on its range, so the
+   * tree is the one the sequential build makes (node numbering aside). */
   void build() {
     const uint32_t n = (uint32_t)boxes.size();
     nodes.reserve(2 * (size_t)n / 2 + 8);
-    struct Task { int32_t node; uint32_t first, count, depth; };
-    std::vector<Task> work;
     nodes.emplace_back();
-    work.push_back({0, 0, n, 0});
-    constexpr int NB = 32;
-    const double C_TRAV = 1.0, C_ISECT = GT_SAH_ISECT;
-    const double lam = GT_SAH_LEAF / GT_SAH_ISECT; /* leaf cost in proxy tests */
+    std::vector<Task> work{{0, 0, n, 0}}, deferred;
+    const unsigned nth = host_threads(n);
+    const uint32_t grain = std::max<uint32_t>(4096u, n / 256u);
     while (!work.empty()) {
-      Task t = work.back();
+      const Task t = work.back();
       work.pop_back();
-      max_depth = std::max(max_depth, t.depth);
-      Box bb, cb;
-      bb.empty();
-      cb.empty();
-      for (uint32_t i = t.first; i < t.first + t.count; ++i) {
-        const uint32_t p = order[i];
-        bb.grow(boxes[p]);
-        Box c;
-        for (int j = 0; j < 3; ++j) c.lo[j] = c.hi[j] = cent[3 * p + j];
-        cb.grow(c);
+      if (nth > 1 && t.count <= grain) {
+        deferred.push_back(t);
+        continue;
       }
-      nodes[t.node].box = bb;
-      auto make_leaf = [&]() {
-        nodes[t.node].left = -1;
-        nodes[t.node].first = t.first;
-        nodes[t.node].count = t.count;
-      };
-      if (t.count <= 1) { make_leaf(); continue; }
-      // best binned SAH split over the three axes
-      int best_axis = -1, best_bin = -1;
-      double best_cost = INFINITY;
-      if (t.depth < 48) {
-        for (int ax = 0; ax < 3; ++ax) {
-          const float ext = cb.hi[ax] - cb.lo[ax];
-          if (!(ext > 0.0f)) continue;
-          Box bin_box[NB];
-          uint32_t bin_cnt[NB] = {0};
-          for (int b = 0; b < NB; ++b) bin_box[b].empty();
-          const float scale = NB / ext;
-          for (uint32_t i = t.first; i < t.first + t.count; ++i) {
-            const uint32_t p = order[i];
-            int b = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale);
-            b = std::min(std::max(b, 0), NB - 1);
-            bin_cnt[b]++;
-            bin_box[b].grow(boxes[p]);
-          }
-          double right_area[NB];
-          uint32_t right_cnt[NB];
-          Box acc;
-          acc.empty();
-          uint32_t cnt = 0;
-          for (int b = NB - 1; b > 0; --b) {
-            acc.grow(bin_box[b]);
-            cnt += bin_cnt[b];
-            right_area[b] = acc.area();
-            right_cnt[b] = cnt;
-          }
-          acc.empty();
-          cnt = 0;
-          for (int b = 0; b < NB - 1; ++b) {
-            acc.grow(bin_box[b]);
-            cnt += bin_cnt[b];
-            if (cnt == 0 || right_cnt[b + 1] == 0) continue;
-            const double c = acc.area() * (cnt + lam) + right_area[b + 1] * (right_cnt[b + 1] + lam);
-            if (c < best_cost) { best_cost = c; best_axis = ax; best_bin = b; }
-          }
+      step(t, nodes, work, max_depth, nth > 1);
+    }
+    if (deferred.empty()) return;
+    std::vector<std::vector<B2Node>> local(deferred.size());
+    std::vector<uint32_t> depth(deferred.size(), 0u);
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (size_t i; (i = next.fetch_add(1)) < deferred.size();) {
+        std::vector<B2Node>& ns = local[i];
+        ns.emplace_back();
+        std::vector<Task> w{{0, deferred[i].first, deferred[i].count, deferred[i].depth}};
+        while (!w.empty()) {
+          const Task t = w.back();
+          w.pop_back();
+          step(t, ns, w, depth[i], false);
         }
       }
-      const double parent_area = std::max(bb.area(), 1e-30);
-      const double split_cost = C_TRAV + C_ISECT * best_cost / parent_area;
-      const double leaf_cost = C_ISECT * (t.count + lam);
-      if (t.count <= GT_LEAF_MAX && (best_axis < 0 || leaf_cost <= split_cost)) { make_leaf(); continue; }
-      uint32_t mid;
-      if (best_axis >= 0) {
-        const int ax = best_axis;
-        const float ext = cb.hi[ax] - cb.lo[ax];
-        const float scale = NB / ext;
-        auto it = std::partition(order.begin() + t.first, order.begin() + t.first + t.count, [&](uint32_t p) {
-          int b = (int)((cent[3 * p + ax] - cb.lo[ax]) * scale);
-          b = std::min(std::max(b, 0), NB - 1);
-          return b <= best_bin;
-        });
-        mid = (uint32_t)(it - order.begin());
-      } else {
-        mid = t.first + t.count;  // force the median fallback below
+    };
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nth; ++k) th.emplace_back(worker);
+    worker();
+    for (auto& x : th) x.join();
+    for (size_t i = 0; i < deferred.size(); ++i) { /* local node 0 is the deferred node itself */
+      const int32_t g0 = deferred[i].node;
+      const int32_t base = (int32_t)nodes.size() - 1;
+      for (size_t k = 0; k < local[i].size(); ++k) {
+        B2Node b = local[i][k];
+        if (b.left >= 0) {
+          b.left = b.left == 0 ? g0 : base + b.left;
+          b.right = b.right == 0 ? g0 : base + b.right;
+        }
+        if (k == 0) nodes[g0] = b;
+        else nodes.push_back(b);
       }
-      if (mid == t.first || mid == t.first + t.count) {
-        // degenerate or too deep: object median on the widest centroid axis
-        int ax = 0;
-        for (int j = 1; j < 3; ++j)
-          if (cb.hi[j] - cb.lo[j] > cb.hi[ax] - cb.lo[ax]) ax = j;
-        mid = t.first + t.count / 2;
-        std::nth_element(order.begin() + t.first, order.begin() + mid, order.begin() + t.first + t.count,
-                         [&](uint32_t a, uint32_t b) {
-                           if (cent[3 * a + ax] != cent[3 * b + ax]) return cent[3 * a + ax] < cent[3 * b + ax];
-                           return a < b;
-                         });
-      }
-      const int32_t l = (int32_t)nodes.size();
-      nodes.emplace_back();
-      const int32_t r = (int32_t)nodes.size();
-      nodes.emplace_back();
-      nodes[t.node].left = l;
-      nodes[t.node].right = r;
-      work.push_back({r, mid, t.first + t.count - mid, t.depth + 1});
-      work.push_back({l, t.first, mid - t.first, t.depth + 1});
+      max_depth = std::max(max_depth, depth[i]);
     }
   }
 };
@@ -470,7 +602,9 @@ static int build_and_upload(gt_scene* s) {
     return GT_E_LIMIT;
   }
   std::vector<Box> boxes(np);
-  for (uint32_t i = 0; i < np; ++i) boxes[i] = proxy_box(&h.o2w[(size_t)12 * i]);
+  parallel_chunks(np, host_threads(np), [&](size_t i0, size_t i1, unsigned) {
+    for (size_t i = i0; i < i1; ++i) boxes[i] = proxy_box(&h.o2w[12 * i]);
+  });
 
   std::vector<gt_node4> nodes;
   std::vector<float> prox((size_t)np * 16, 0.0f);
