@@ -142,6 +142,20 @@ class MemSampler:
         self.peak = max(self.peak, total - free)
 
 
+def measured_copy_gbs(torch, stream, nbytes=1 << 30, reps=5):
+    """HBM bandwidth of a plain 1 GiB device-to-device copy on this box
+    (bytes read + written per second), the measured counterpart of the
+    nominal peak."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ms = _events_ms(torch, stream, lambda: b.copy_(a), reps)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def _events_ms(torch, stream, fn, n):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -492,6 +506,7 @@ def main():
     B_fwd = 16 * R + 44 * st["h_eval"] + 192 * st["h_comp"]
     achieved = B_fwd / (ms_step * 1e-3) / 1e9
     img = rgba.cpu().numpy()
+    copy_gbs = measured_copy_gbs(torch, stream)
     out = {
         "metric": METRIC,
         "value": round(mrays, 3),
@@ -517,6 +532,7 @@ def main():
             "traffic": (TRAFFIC_DEFAULT_FRAME["fetch_bytes"] + TRAFFIC_DEFAULT_FRAME["write_bytes"]
                         if (args.n, W, H, K) == (1000000, 1920, 1080, 16) else None),
             "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16, 0, false, false, false, true> (packet traversal)",
+            "measured_copy_gbs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5),
         },
         "stats": st,
         "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
