@@ -688,10 +688,11 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
                                                  const float4* __restrict__ lbox,
                                                  const uint32_t* __restrict__ items, unsigned int* ctl,
                                                  uint32_t* __restrict__ perm, uint32_t bn_cap) {
-  __shared__ float s_lo[2][3][PLOC_N], s_hi[2][3][PLOC_N];
-  __shared__ float s_D[2][4][PLOC_N]; /* collapse costs of the clusters (dp_node) */
-  __shared__ float s_bc[2][PLOC_N];   /* binary costs of the clusters */
-  __shared__ uint32_t s_ref[2][PLOC_N], s_cnt[2][PLOC_N];
+  /* the cluster list, compacted in place each round */
+  __shared__ float s_lo[3][PLOC_N], s_hi[3][PLOC_N];
+  __shared__ float s_D[4][PLOC_N]; /* collapse costs of the clusters (dp_node) */
+  __shared__ float s_bc[PLOC_N];   /* binary costs of the clusters */
+  __shared__ uint32_t s_ref[PLOC_N], s_cnt[PLOC_N];
   __shared__ uint32_t s_nn[PLOC_N];
   __shared__ float s_blo[3][PLOC_N], s_bhi[3][PLOC_N]; /* made nodes, by local slot */
   __shared__ float s_bD[4][PLOC_N], s_bbc[PLOC_N];
@@ -717,47 +718,69 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
     }
     if (tid < count) {
       const float4 a = lbox[2 * (size_t)(first + tid)], b = lbox[2 * (size_t)(first + tid) + 1];
-      s_lo[0][0][tid] = a.x; s_lo[0][1][tid] = a.y; s_lo[0][2][tid] = a.z;
-      s_hi[0][0][tid] = b.x; s_hi[0][1][tid] = b.y; s_hi[0][2][tid] = b.z;
+      s_lo[0][tid] = a.x; s_lo[1][tid] = a.y; s_lo[2][tid] = a.z;
+      s_hi[0][tid] = b.x; s_hi[1][tid] = b.y; s_hi[2][tid] = b.z;
       const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
       const float c = ((float)GT_SAH_ISECT + (float)GT_SAH_LEAF) * (float)box_area(lo, hi);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) s_D[0][q][tid] = c;
-      s_bc[0][tid] = c;
-      s_ref[0][tid] = PLOC_NODE_LEAF | tid;
-      s_cnt[0][tid] = 1u;
+      for (int q = 0; q < 4; ++q) s_D[q][tid] = c;
+      s_bc[tid] = c;
+      s_ref[tid] = PLOC_NODE_LEAF | tid;
+      s_cnt[tid] = 1u;
     }
     __syncthreads();
     const uint32_t base = s_base;
     const bool fits = (np - 1u) + base + (count > 2u ? count - 2u : 0u) <= bn_cap;
-    uint32_t n = count, cur = 0, nit = 0;
+    uint32_t n = count, nit = 0;
     while (n > 1u && fits) {
       /* nearest neighbour in the window */
       if (tid < n) {
-        const float alo[3] = {s_lo[cur][0][tid], s_lo[cur][1][tid], s_lo[cur][2][tid]};
-        const float ahi[3] = {s_hi[cur][0][tid], s_hi[cur][1][tid], s_hi[cur][2][tid]};
+        const float alo[3] = {s_lo[0][tid], s_lo[1][tid], s_lo[2][tid]};
+        const float ahi[3] = {s_hi[0][tid], s_hi[1][tid], s_hi[2][tid]};
         const uint32_t jlo = tid > PLOC_RADIUS ? tid - PLOC_RADIUS : 0u;
         const uint32_t jhi = min(tid + PLOC_RADIUS, n - 1u);
         float best = INFINITY;
         uint32_t bj = tid;
         for (uint32_t j = jlo; j <= jhi; ++j) {
           if (j == tid) continue;
-          const float dx = fmaxf(ahi[0], s_hi[cur][0][j]) - fminf(alo[0], s_lo[cur][0][j]);
-          const float dy = fmaxf(ahi[1], s_hi[cur][1][j]) - fminf(alo[1], s_lo[cur][1][j]);
-          const float dz = fmaxf(ahi[2], s_hi[cur][2][j]) - fminf(alo[2], s_lo[cur][2][j]);
+          const float dx = fmaxf(ahi[0], s_hi[0][j]) - fminf(alo[0], s_lo[0][j]);
+          const float dy = fmaxf(ahi[1], s_hi[1][j]) - fminf(alo[1], s_lo[1][j]);
+          const float dz = fmaxf(ahi[2], s_hi[2][j]) - fminf(alo[2], s_lo[2][j]);
           const float a = dx * dy + dy * dz + dz * dx;
           if (a < best || bj == tid) { best = a; bj = j; }
         }
         s_nn[tid] = bj;
       }
       __syncthreads();
+      /* this cluster and, for a merge, its partner, read into registers
+       * before the barrier that precedes the in-place compaction */
       bool keep = false, make = false;
-      uint32_t j = tid;
+      float mlo[3] = {0.f, 0.f, 0.f}, mhi[3] = {0.f, 0.f, 0.f}, mD[4] = {0.f, 0.f, 0.f, 0.f}, mbc = 0.f;
+      float plo[3] = {0.f, 0.f, 0.f}, phi[3] = {0.f, 0.f, 0.f}, pD[4] = {0.f, 0.f, 0.f, 0.f}, pbc = 0.f;
+      uint32_t mref = 0u, mcnt = 0u, pref = 0u, pcnt = 0u;
       if (tid < n) {
-        j = s_nn[tid];
+        const uint32_t j = s_nn[tid];
         const bool mutual = j != tid && s_nn[j] == tid;
         keep = !mutual || tid < j;
         make = mutual && tid < j;
+        if (keep) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) { mlo[q] = s_lo[q][tid]; mhi[q] = s_hi[q][tid]; }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) mD[q] = s_D[q][tid];
+          mbc = s_bc[tid];
+          mref = s_ref[tid];
+          mcnt = s_cnt[tid];
+        }
+        if (make) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) { plo[q] = s_lo[q][j]; phi[q] = s_hi[q][j]; }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pD[q] = s_D[q][j];
+          pbc = s_bc[j];
+          pref = s_ref[j];
+          pcnt = s_cnt[j];
+        }
       }
       /* exclusive scan of keep over the block */
       const uint64_t km = __ballot(keep);
@@ -768,49 +791,43 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
         if (w < wv) pos += s_wsum[w];
         total += s_wsum[w];
       }
-      const uint32_t nxt = cur ^ 1u;
       if (make) {
         const uint32_t l = atomicAdd(&s_made, 1u);
         float lo[3], hi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          lo[q] = fminf(s_lo[cur][q][tid], s_lo[cur][q][j]);
-          hi[q] = fmaxf(s_hi[cur][q][tid], s_hi[cur][q][j]);
+          lo[q] = fminf(mlo[q], plo[q]);
+          hi[q] = fmaxf(mhi[q], phi[q]);
         }
-        const uint32_t cnt = s_cnt[cur][tid] + s_cnt[cur][j];
-        float dl[4], dr[4], D[4], bc;
+        const uint32_t cnt = mcnt + pcnt;
+        float D[4], bc;
         uint8_t chs[4];
         uint32_t leaf;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { dl[q] = s_D[cur][q][tid]; dr[q] = s_D[cur][q][j]; }
-        dp_node(dl, dr, s_bc[cur][tid], s_bc[cur][j], cnt, (float)box_area(lo, hi), D, chs, &leaf, &bc);
+        dp_node(mD, pD, mbc, pbc, cnt, (float)box_area(lo, hi), D, chs, &leaf, &bc);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           s_blo[q][l] = lo[q]; s_bhi[q][l] = hi[q];
-          s_lo[nxt][q][pos] = lo[q]; s_hi[nxt][q][pos] = hi[q];
+          s_lo[q][pos] = lo[q]; s_hi[q][pos] = hi[q];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { s_bD[q][l] = D[q]; s_D[nxt][q][pos] = D[q]; }
+        for (int q = 0; q < 4; ++q) { s_bD[q][l] = D[q]; s_D[q][pos] = D[q]; }
         s_bbc[l] = bc;
-        s_bc[nxt][pos] = bc;
+        s_bc[pos] = bc;
         s_bch[l] = (uint32_t)chs[0] | ((uint32_t)chs[1] << 8) | ((uint32_t)chs[2] << 16) | ((uint32_t)chs[3] << 24);
-        s_bl[l] = s_ref[cur][tid];
-        s_br[l] = s_ref[cur][j];
+        s_bl[l] = mref;
+        s_br[l] = pref;
         s_bcnt[l] = cnt;
         s_bleaf[l] = leaf;
-        s_ref[nxt][pos] = l;
-        s_cnt[nxt][pos] = cnt;
+        s_ref[pos] = l;
+        s_cnt[pos] = cnt;
       } else if (keep) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          s_lo[nxt][q][pos] = s_lo[cur][q][tid];
-          s_hi[nxt][q][pos] = s_hi[cur][q][tid];
-        }
-        s_ref[nxt][pos] = s_ref[cur][tid];
-        s_cnt[nxt][pos] = s_cnt[cur][tid];
+        for (int q = 0; q < 3; ++q) { s_lo[q][pos] = mlo[q]; s_hi[q][pos] = mhi[q]; }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s_D[nxt][q][pos] = s_D[cur][q][tid];
-        s_bc[nxt][pos] = s_bc[cur][tid];
+        for (int q = 0; q < 4; ++q) s_D[q][pos] = mD[q];
+        s_bc[pos] = mbc;
+        s_ref[pos] = mref;
+        s_cnt[pos] = mcnt;
       }
       __syncthreads();
       if (total >= n) { /* no merge: only with NaN boxes */
@@ -819,7 +836,6 @@ __global__ __launch_bounds__(PLOC_N) void k_ploc(uint32_t np, BNode* __restrict_
       }
       if (tid == 0) s_itend[nit] = s_made;
       n = total;
-      cur = nxt;
       ++nit;
     }
     __syncthreads();
