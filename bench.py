@@ -452,7 +452,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(rays shard with no exchange; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "(rays would shard over GPUs by tile ranges with no exchange, each GPU holding the whole scene)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
