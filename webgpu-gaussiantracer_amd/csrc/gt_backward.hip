@@ -95,7 +95,7 @@ __device__ __forceinline__ void sh_basis(double x, double y, double z, double* b
 #define GT_ACC 16
 
 struct Gauss {
-  double mu[3], s[3], q[4], qn, op;
+  double mu[3], is[3], q[4], qn, op; /* is: 1 / scale, one divide per load */
   double A[3][3]; /* A[r][c]: applied rotation (GLSL R[c][r]) */
 };
 
@@ -103,7 +103,7 @@ __device__ __forceinline__ void load_gauss(const gt_dev_scene& S, uint32_t id, G
   const float4* G = reinterpret_cast<const float4*>(S.geom) + 3 * (size_t)id;
   const float4 g0 = G[0], g1 = G[1], g2 = G[2];
   g.mu[0] = g0.x; g.mu[1] = g0.y; g.mu[2] = g0.z; g.op = g0.w;
-  g.s[0] = g1.x; g.s[1] = g1.y; g.s[2] = g1.z; g.qn = g1.w;
+  g.is[0] = 1.0 / (double)g1.x; g.is[1] = 1.0 / (double)g1.y; g.is[2] = 1.0 / (double)g1.z; g.qn = g1.w;
   g.q[0] = g2.x; g.q[1] = g2.y; g.q[2] = g2.z; g.q[3] = g2.w;
   const double r = g.q[0], x = g.q[1], y = g.q[2], z = g.q[3];
   g.A[0][0] = 1 - 2 * (y * y + z * z); g.A[0][1] = 2 * (x * y + r * z); g.A[0][2] = 2 * (x * z - r * y);
@@ -111,23 +111,24 @@ __device__ __forceinline__ void load_gauss(const gt_dev_scene& S, uint32_t id, G
   g.A[2][0] = 2 * (x * z + r * y); g.A[2][1] = 2 * (y * z - r * x); g.A[2][2] = 1 - 2 * (x * x + y * y);
 }
 
-/* whitened-frame response of one hit: a, b, c = a x b, |b|^2, e, exp(-e) */
+/* whitened-frame response of one hit: a, b, c = a x b, 1 / |b|^2, e, exp(-e) */
 struct Resp {
-  double a[3], b[3], cr[3], bb, e, resp;
+  double a[3], b[3], cr[3], ibb, e, resp;
 };
 
 __device__ __forceinline__ void eval_resp(const Gauss& g, const double* o, const double* d, Resp& h) {
+#pragma clang fp contract(fast)
   double u[3];
   for (int i = 0; i < 3; ++i) u[i] = o[i] - g.mu[i];
   for (int rr = 0; rr < 3; ++rr) {
-    h.a[rr] = (g.A[rr][0] * u[0] + g.A[rr][1] * u[1] + g.A[rr][2] * u[2]) / g.s[rr];
-    h.b[rr] = (g.A[rr][0] * d[0] + g.A[rr][1] * d[1] + g.A[rr][2] * d[2]) / g.s[rr];
+    h.a[rr] = (g.A[rr][0] * u[0] + g.A[rr][1] * u[1] + g.A[rr][2] * u[2]) * g.is[rr];
+    h.b[rr] = (g.A[rr][0] * d[0] + g.A[rr][1] * d[1] + g.A[rr][2] * d[2]) * g.is[rr];
   }
   h.cr[0] = h.a[1] * h.b[2] - h.a[2] * h.b[1];
   h.cr[1] = h.a[2] * h.b[0] - h.a[0] * h.b[2];
   h.cr[2] = h.a[0] * h.b[1] - h.a[1] * h.b[0];
-  h.bb = h.b[0] * h.b[0] + h.b[1] * h.b[1] + h.b[2] * h.b[2];
-  h.e = (h.cr[0] * h.cr[0] + h.cr[1] * h.cr[1] + h.cr[2] * h.cr[2]) / h.bb;
+  h.ibb = 1.0 / (h.b[0] * h.b[0] + h.b[1] * h.b[1] + h.b[2] * h.b[2]);
+  h.e = (h.cr[0] * h.cr[0] + h.cr[1] * h.cr[1] + h.cr[2] * h.cr[2]) * h.ibb;
   h.resp = gt_exp64(-h.e);
 }
 
@@ -156,6 +157,41 @@ __device__ __forceinline__ int table_slot(uint32_t* key, uint32_t id) {
   return -1;
 }
 
+/* The value of lane ^ X, for X in {1, 2, 8, 16, 32}.  Within a row of 16
+ * lanes DPP moves: quad_perm [1,0,3,2] (^1), quad_perm [2,3,0,1] (^2) and
+ * row_ror:8 (^8).  Across rows the gfx950 swaps: v_permlane16_swap_b32
+ * with both operands x gives x[row 0] in odd rows' first result and
+ * x[row 1] in even rows' second result (^16); v_permlane32_swap_b32 the
+ * same for halves (^32). */
+template <uint32_t X>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x, uint32_t lane) {
+  if constexpr (X == 1u) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+  else if constexpr (X == 2u) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, true);
+  else if constexpr (X == 8u) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, true);
+  else if constexpr (X == 16u) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (lane & 16u) ? r[0] : r[1];
+  } else {
+    static_assert(X == 32u, "lane_xor: unsupported distance");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (lane & 32u) ? r[0] : r[1];
+  }
+}
+/* Merges lane L with lane L ^ X when both hold the same key: the lower lane
+ * adds the partner's values, the upper one clears its key.  Called with
+ * every lane of the wave active. */
+template <uint32_t X>
+__device__ __forceinline__ void merge_pair(uint32_t lane, uint32_t& key, float* v) {
+  const uint32_t pk = lane_xor<X>(key, lane);
+  const bool same = pk == key && key != BW_EMPTY;
+  const bool upper = (lane & X) != 0u;
+#pragma unroll
+  for (int k = 0; k < BW_VALS; ++k) {
+    const float pv = __uint_as_float(lane_xor<X>(__float_as_uint(v[k]), lane));
+    if (same && !upper) v[k] += pv;
+  }
+  if (same && upper) key = BW_EMPTY;
+}
 /* One thread per ray; a block covers 16x16 pixels of rows [y0, y1) as four
  * 8x8 wave tiles.  Two passes over the ray's composited ids, in FP64: the
  * first accumulates the pixel L, the second walks the hits front to back
@@ -164,6 +200,7 @@ __global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const
                                                   const float4* __restrict__ dl_drgba,
                                                   const float4* __restrict__ colors, uint32_t y0, uint32_t y1,
                                                   float* __restrict__ acc, unsigned int* overflow) {
+#pragma clang fp contract(fast)
   __shared__ uint32_t s_key[BW_TABLE];
   __shared__ float s_val[BW_VALS * BW_TABLE];
   for (uint32_t i = threadIdx.x; i < BW_TABLE; i += 256u) {
@@ -172,14 +209,6 @@ __global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const
     for (int k = 0; k < BW_VALS; ++k) s_val[k * BW_TABLE + i] = 0.0f;
   }
   __syncthreads();
-  /* accumulator k of Gaussian id: the block's LDS entry, else HBM */
-  auto add = [&](int e, uint32_t id, int k, double v) {
-    if (v == 0.0) return;
-    if (e >= 0)
-      atomicAdd(&s_val[k * BW_TABLE + (uint32_t)e], (float)v);
-    else
-      unsafeAtomicAdd(acc + (size_t)id * GT_ACC + k, (float)v);
-  };
   const uint32_t bx_n = (F.width + 15u) / 16u;
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t px = (blockIdx.x % bx_n) * 16u + (w & 1u) * 8u + (lane & 7u);
@@ -197,31 +226,39 @@ __global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const
     if (n) gr = dl_drgba[pix];
   }
   const double G[3] = {gr.x, gr.y, gr.z};
-  if (n && (G[0] != 0.0 || G[1] != 0.0 || G[2] != 0.0)) {
-    const gt_f3 df = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
-    const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
-    const double d[3] = {df.x, df.y, df.z};
-    const uint32_t* ids = F.comp_ids + (size_t)slot * F.comp_cap;
-    /* pass 1: L */
-    double L[3] = {0, 0, 0}, T = 1.0;
-    for (uint32_t i = 0; i < n; ++i) {
+  if (G[0] == 0.0 && G[1] == 0.0 && G[2] == 0.0) n = 0;
+  const gt_f3 df = gt_ray_dir(F.to_world, F.ax, F.ay, px, py, F.width, F.height);
+  const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
+  const double d[3] = {df.x, df.y, df.z};
+  const uint32_t* ids = F.comp_ids + (size_t)slot * F.comp_cap;
+  /* pass 1: L */
+  double L[3] = {0, 0, 0}, T = 1.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t id = ids[i];
+    Gauss g;
+    load_gauss(S, id, g);
+    Resp h;
+    eval_resp(g, o, d, h);
+    const double alpha = h.resp * g.op;
+    const float4 c = colors[id];
+    L[0] += T * alpha * c.x;
+    L[1] += T * alpha * c.y;
+    L[2] += T * alpha * c.z;
+    T *= 1.0 - alpha;
+  }
+  /* pass 2: adjoints front to back.  The loop runs in step across the
+   * wave (lanes past their last hit idle), so that the contributions of
+   * lanes holding the same Gaussian can be merged before the atomics. */
+  double Lp[3] = {0, 0, 0};
+  T = 1.0;
+  for (uint32_t i = 0; __ballot(i < n) != 0ull; ++i) {
+    float v[BW_VALS];
+#pragma unroll
+    for (int k = 0; k < BW_VALS; ++k) v[k] = 0.0f;
+    uint32_t key = BW_EMPTY;
+    if (i < n) {
       const uint32_t id = ids[i];
-      Gauss g;
-      load_gauss(S, id, g);
-      Resp h;
-      eval_resp(g, o, d, h);
-      const double alpha = h.resp * g.op;
-      const float4 c = colors[id];
-      L[0] += T * alpha * c.x;
-      L[1] += T * alpha * c.y;
-      L[2] += T * alpha * c.z;
-      T *= 1.0 - alpha;
-    }
-    /* pass 2: adjoints front to back */
-    double Lp[3] = {0, 0, 0};
-    T = 1.0;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t id = ids[i];
+      key = id;
       Gauss g;
       load_gauss(S, id, g);
       Resp h;
@@ -230,18 +267,18 @@ __global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const
       const float4 c4 = colors[id];
       const double c[3] = {c4.x, c4.y, c4.z};
       const double one_m = 1.0 - alpha;
+      const double i_one_m = one_m > 0.0 ? 1.0 / one_m : 0.0;
       double dL_dalpha = 0.0;
-      const int e = table_slot(s_key, id);
       for (int ch = 0; ch < 3; ++ch) {
         Lp[ch] += T * alpha * c[ch];
         const double rest = L[ch] - Lp[ch];
-        dL_dalpha += G[ch] * (T * c[ch] - (one_m > 0.0 ? rest / one_m : 0.0));
-        add(e, id, ch, G[ch] * T * alpha);
+        dL_dalpha += G[ch] * (T * c[ch] - rest * i_one_m);
+        v[ch] = (float)(G[ch] * T * alpha);
       }
-      add(e, id, 3, dL_dalpha * h.resp);
+      v[3] = (float)(dL_dalpha * h.resp);
       /* response: resp = exp(-e), e = |a x b|^2 / |b|^2 */
       const double dL_de = -h.resp * g.op * dL_dalpha;
-      const double k2 = 2.0 / h.bb;
+      const double k2 = 2.0 * h.ibb;
       const double bxc[3] = {h.b[1] * h.cr[2] - h.b[2] * h.cr[1], h.b[2] * h.cr[0] - h.b[0] * h.cr[2],
                              h.b[0] * h.cr[1] - h.b[1] * h.cr[0]};
       const double cxa[3] = {h.cr[1] * h.a[2] - h.cr[2] * h.a[1], h.cr[2] * h.a[0] - h.cr[0] * h.a[2],
@@ -254,28 +291,49 @@ __global__ __launch_bounds__(256, 4) void k_backward(const gt_dev_scene S, const
       /* a = S^-1 A (o - mu), b = S^-1 A d */
       double M[3][3];
       for (int r = 0; r < 3; ++r) {
-        const double is = 1.0 / g.s[r];
+        const double is = g.is[r];
         double sum = 0.0;
         for (int cc = 0; cc < 3; ++cc) {
           const double t = dL_da[r] * (o[cc] - g.mu[cc]) + dL_db[r] * d[cc];
           M[r][cc] = is * t;
           sum += g.A[r][cc] * t;
         }
-        add(e, id, 7 + r, -is * sum); /* d(1/s)/dlog s = -1/s */
+        v[7 + r] = (float)(-is * sum); /* d(1/s)/dlog s = -1/s */
       }
       for (int cc = 0; cc < 3; ++cc)
-        add(e, id, 4 + cc, -(dL_da[0] * g.A[0][cc] / g.s[0] + dL_da[1] * g.A[1][cc] / g.s[1] +
-                          dL_da[2] * g.A[2][cc] / g.s[2]));
+        v[4 + cc] = (float)(-(dL_da[0] * g.A[0][cc] * g.is[0] + dL_da[1] * g.A[1][cc] * g.is[1] +
+                              dL_da[2] * g.A[2][cc] * g.is[2]));
       /* quaternion (r, x, y, z) of A */
       const double qr = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
-      add(e, id, 10, 2 * (qz * M[0][1] - qy * M[0][2] - qz * M[1][0] + qx * M[1][2] + qy * M[2][0] - qx * M[2][1]));
-      add(e, id, 11, 2 * (qy * M[0][1] + qz * M[0][2] + qy * M[1][0] - 2 * qx * M[1][1] + qr * M[1][2] + qz * M[2][0] -
-                       qr * M[2][1] - 2 * qx * M[2][2]));
-      add(e, id, 12, 2 * (-2 * qy * M[0][0] + qx * M[0][1] - qr * M[0][2] + qx * M[1][0] + qz * M[1][2] + qr * M[2][0] +
-                       qz * M[2][1] - 2 * qy * M[2][2]));
-      add(e, id, 13, 2 * (-2 * qz * M[0][0] + qr * M[0][1] + qx * M[0][2] - qr * M[1][0] - 2 * qz * M[1][1] +
-                       qy * M[1][2] + qx * M[2][0] + qy * M[2][1]));
+      v[10] = (float)(2 * (qz * M[0][1] - qy * M[0][2] - qz * M[1][0] + qx * M[1][2] + qy * M[2][0] - qx * M[2][1]));
+      v[11] = (float)(2 * (qy * M[0][1] + qz * M[0][2] + qy * M[1][0] - 2 * qx * M[1][1] + qr * M[1][2] +
+                           qz * M[2][0] - qr * M[2][1] - 2 * qx * M[2][2]));
+      v[12] = (float)(2 * (-2 * qy * M[0][0] + qx * M[0][1] - qr * M[0][2] + qx * M[1][0] + qz * M[1][2] +
+                           qr * M[2][0] + qz * M[2][1] - 2 * qy * M[2][2]));
+      v[13] = (float)(2 * (-2 * qz * M[0][0] + qr * M[0][1] + qx * M[0][2] - qr * M[1][0] - 2 * qz * M[1][1] +
+                           qy * M[1][2] + qx * M[2][0] + qy * M[2][1]));
       T *= one_m;
+    }
+    /* Neighbouring rays composite the same Gaussian at the same step, and
+     * same-address LDS atomics serialise.  Lanes (x, y) = (lane & 7,
+     * lane >> 3) of the 8x8 ray tile merge with their partners at x ^ 1,
+     * x ^ 2, y ^ 1, y ^ 2 and y ^ 4 when the keys match: the lower lane
+     * takes the sum and the upper one drops out (key BW_EMPTY). */
+    merge_pair<1u>(lane, key, v);
+    merge_pair<2u>(lane, key, v);
+    merge_pair<8u>(lane, key, v);
+    merge_pair<16u>(lane, key, v);
+    merge_pair<32u>(lane, key, v);
+    if (key != BW_EMPTY) {
+      const int e = table_slot(s_key, key);
+#pragma unroll
+      for (int k = 0; k < BW_VALS; ++k) {
+        if (v[k] == 0.0f) continue;
+        if (e >= 0)
+          atomicAdd(&s_val[k * BW_TABLE + (uint32_t)e], v[k]);
+        else
+          unsafeAtomicAdd(acc + (size_t)key * GT_ACC + k, v[k]);
+      }
     }
   }
   __syncthreads();
