@@ -353,23 +353,13 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* __restrict__ p, size_t 
   for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) p[i] = 0.0f;
 }
 
-/* One thread per Gaussian: the chain rule from the accumulators to the raw
- * PLY parameters (mean, f_dc/f_rest, opacity logit, log-scale, raw
- * quaternion), added into grads (n x 59). */
-__global__ __launch_bounds__(256) void k_finalize(const gt_dev_scene S, const gt_dev_frame F,
-                                                  const float* __restrict__ acc, float* __restrict__ grads) {
-  const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-  if (id >= S.n) return;
-  const float4* A4 = reinterpret_cast<const float4*>(acc + (size_t)id * GT_ACC);
-  const float4 a0 = A4[0], a1 = A4[1], a2 = A4[2], a3 = A4[3];
-  const float ac[GT_ACC] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
-                            a2.x, a2.y, a2.z, a2.w, a3.x, a3.y, a3.z, a3.w};
-  bool any = false;
-  for (int i = 0; i < GT_ACC; ++i) any |= ac[i] != 0.0f;
-  if (!any) return;
+/* The chain rule of one Gaussian from its accumulators ac to the raw PLY
+ * parameters (mean, f_dc/f_rest, opacity logit, log-scale, raw quaternion),
+ * added into its gradient row out (59 floats). */
+__device__ __forceinline__ void finalize_one(const gt_dev_scene& S, const gt_dev_frame& F, uint32_t id,
+                                             const float* ac, float* out) {
   Gauss g;
   load_gauss(S, id, g);
-  float* out = grads + (size_t)id * GT_RAW_FLOATS;
   const double o[3] = {F.to_world[12], F.to_world[13], F.to_world[14]};
   const double u[3] = {o[0] - g.mu[0], o[1] - g.mu[1], o[2] - g.mu[2]};
   /* colour: SH coefficients and the direction normalize(mu - o) */
@@ -412,6 +402,42 @@ __global__ __launch_bounds__(256) void k_finalize(const gt_dev_scene S, const gt
   const double dq[4] = {ac[10], ac[11], ac[12], ac[13]};
   const double qd = dq[0] * g.q[0] + dq[1] * g.q[1] + dq[2] * g.q[2] + dq[3] * g.q[3];
   for (int k = 0; k < 4; ++k) out[55 + k] += (float)((dq[k] - g.q[k] * qd) / g.qn);
+}
+
+/* One thread per Gaussian, FIN_BLOCK Gaussians per block.  The block's
+ * gradient rows (FIN_BLOCK x 236 B, contiguous) are copied to LDS with
+ * coalesced loads, updated there, and written back the same way; a thread
+ * updating its own row in HBM made each access instruction touch 64 rows
+ * 236 B apart.  Blocks whose Gaussians have no accumulators skip the copy.
+ * (Staging the SH records in LDS as well halved the blocks per CU and
+ * doubled the time.) */
+#define FIN_BLOCK 128u
+__global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const gt_dev_scene S, const gt_dev_frame F,
+                                                        const float* __restrict__ acc, float* __restrict__ grads) {
+  __shared__ float s_g[FIN_BLOCK * GT_RAW_FLOATS];
+  const uint32_t base = blockIdx.x * FIN_BLOCK;
+  const uint32_t cnt = S.n - base < FIN_BLOCK ? S.n - base : FIN_BLOCK;
+  const uint32_t id = base + threadIdx.x;
+  float ac[GT_ACC] = {};
+  bool any = false;
+  if (threadIdx.x < cnt) {
+    const float4* A4 = reinterpret_cast<const float4*>(acc + (size_t)id * GT_ACC);
+    const float4 a0 = A4[0], a1 = A4[1], a2 = A4[2], a3 = A4[3];
+    const float v[GT_ACC] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                             a2.x, a2.y, a2.z, a2.w, a3.x, a3.y, a3.z, a3.w};
+    for (int i = 0; i < GT_ACC; ++i) {
+      ac[i] = v[i];
+      any |= v[i] != 0.0f;
+    }
+  }
+  if (!__syncthreads_or(any)) return;
+  float* const gblk = grads + (size_t)base * GT_RAW_FLOATS;
+  const uint32_t nf = cnt * GT_RAW_FLOATS;
+  for (uint32_t j = threadIdx.x; j < nf; j += FIN_BLOCK) s_g[j] = gblk[j];
+  __syncthreads();
+  if (any) finalize_one(S, F, id, ac, s_g + threadIdx.x * GT_RAW_FLOATS);
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nf; j += FIN_BLOCK) gblk[j] = s_g[j];
 }
 
 __global__ void k_zero_u32(uint32_t* p) {
@@ -474,7 +500,8 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
     return hipGetLastError();
   };
   auto finalize = [&]() -> hipError_t {
-    hipLaunchKernelGGL(k_finalize, dim3((S->n + 255u) / 256u), dim3(256), 0, st, *S, f, (const float*)acc, grads);
+    hipLaunchKernelGGL(k_finalize, dim3((S->n + FIN_BLOCK - 1u) / FIN_BLOCK), dim3(FIN_BLOCK), 0, st, *S, f,
+                       (const float*)acc, grads);
     return hipGetLastError();
   };
   if (comp_ids) {

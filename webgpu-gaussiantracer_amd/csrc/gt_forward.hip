@@ -846,13 +846,25 @@ __global__ void k_reset_counter(unsigned int* work) { *work = 0u; }
 /* Per-Gaussian radiance seen from one camera origin (the direction
  * normalize(mu - o) of GS_shader.ts:264 depends on the Gaussian and the
  * origin only): the same FP32 evaluation as the render kernels. */
+/* One thread per Gaussian.  The block's SH records (256 x 192 B,
+ * contiguous) are first copied to LDS with coalesced 16-B loads, padded to
+ * 13 float4 per record so that the per-thread reads do not conflict; a
+ * thread reading its own record straight from HBM made each load
+ * instruction touch 64 records 192 B apart. */
+#define COLORS_PAD 13u
 __global__ __launch_bounds__(256) void k_colors(const gt_dev_scene S, float ox, float oy, float oz, uint32_t deg,
                                                 float4* __restrict__ out) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  __shared__ float4 s_sh[256u * COLORS_PAD];
+  const uint32_t base = blockIdx.x * 256u;
+  const uint32_t cnt = S.n - base < 256u ? S.n - base : 256u;
+  const float4* src = reinterpret_cast<const float4*>(S.sh) + 12u * (size_t)base;
+  for (uint32_t j = threadIdx.x; j < cnt * 12u; j += 256u) s_sh[(j / 12u) * COLORS_PAD + j % 12u] = src[j];
+  __syncthreads();
+  const uint32_t i = base + threadIdx.x;
   if (i >= S.n) return;
   const float4 g0 = reinterpret_cast<const float4*>(S.geom)[3 * (size_t)i];
   const gt_f3 dir = gt_normalize(gt_sub(gt_f3_make(g0.x, g0.y, g0.z), gt_f3_make(ox, oy, oz)));
-  const gt_f3 c = radiance(S.sh + 48 * (size_t)i, deg, dir);
+  const gt_f3 c = radiance(reinterpret_cast<const float*>(s_sh + threadIdx.x * COLORS_PAD), deg, dir);
   out[i] = make_float4(c.x, c.y, c.z, 0.0f);
 }
 
