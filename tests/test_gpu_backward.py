@@ -167,6 +167,37 @@ def test_backward_vs_central_differences():
     assert np.all(np.abs(av - fd) <= tol)
 
 
+def test_backward_shared_gaussians_ragged_tiles():
+    """Gaussian 0 lies in front of the camera and covers the view, so every
+    ray composites it first; Gaussian 1 covers about half of it.  The lanes
+    of a wave then hold the same Gaussian at the same step, which is the case
+    where the kernel merges lanes before its LDS atomics.  45x37 leaves
+    ragged 8x8 tiles at the right and bottom edges."""
+    _print("case backward shared Gaussians, ragged tiles 45x37")
+    raw = gt.generate_raw(n=2000, seed=5, config="c1", rest_std=0.2)
+    raw[0, 0:3] = [0.25, -0.15, 2.8]
+    raw[0, 52:55] = np.log(0.8)
+    raw[0, 51] = 0.0
+    raw[0, 55:59] = [1.0, 0.0, 0.0, 0.0]
+    raw[1, 0:3] = [0.9, -0.3, 2.6]
+    raw[1, 52:55] = np.log([0.5, 0.8, 0.4])
+    raw[1, 51] = 0.5
+    raw[1, 55:59] = [0.9, 0.2, -0.3, 0.1]
+    W, H = 45, 37
+    s = gt.Scene(raw, alpha_min=0.05)
+    u = uniforms([0.3, -0.2, 5.0], [0, 0, 0], W, H, hfov_deg=50, alpha_min=0.05, k=8, T_min=1e-3, sh_degree_max=3)
+    G = np.random.default_rng(5).standard_normal((H, W, 4)).astype(np.float32)
+    G[..., 3] = 0.0
+    _, ref, ck, _ = _reference(raw, s, u, W, H, G, 3)
+    first = ck["comp_ids"][:, 0][ck["n_comp"] > 0]
+    both = np.mean([1 in ck["comp_ids"][r, :ck["n_comp"][r]] for r in range(W * H)])
+    _print(f"  rays with Gaussian 0 first: {np.mean(first == 0):.3f}, with Gaussian 1: {both:.3f}")
+    assert np.all(first == 0) and 0.2 < both < 0.8
+    g, ovf = _gpu_grads(s, raw.shape[0], u, W, H, G)
+    assert ovf == 0
+    _check_grads(g, ref, "shared")
+
+
 def test_recompute_equals_checkpoint():
     """Recompute mode over several bands (small scratch budget) == checkpoint mode up to atomic order."""
     _print("case backward recompute vs checkpoint")
