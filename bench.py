@@ -21,8 +21,9 @@ loss + the hand-written backward, on the same frame (and the recompute mode).
 batched: 64 cameras of uneven resolution in one launch replayed from a
 hipGraph, against the same cameras rendered one launch at a time.
 fwd_bwd_c4: the same forward + backward on 200k single-cluster Gaussians, 800x800.
-c5: 3M Gaussians, 16 views at 1280x720, forward + recompute-mode backward
-per view with gradients accumulated across views, peak device memory.
+c5: 3M Gaussians, 16 views at 1280x720, forward with checkpoint + backward
+per view with gradients accumulated across views, peak device memory (and
+the recompute-mode step beside it).
 device_build: the same 1M-Gaussian scene built on the GPU from device
 parameters (rebuild time) and the forward frame time on that tree.
 train_c4: one training step (forward with checkpoint, backward, Adam, scene
@@ -284,10 +285,13 @@ def bench_c4(gt, torch, dev, sp, steps):
 def bench_c5(gt, torch, dev, sp, steps):
     """Batched forward + backward at scale: 3M Gaussians (256-cluster mixture in
     a 40^3 box, log-scales N(-4.5, 0.7)), 16 cameras at 1280x720 on a sphere of
-    radius 25 (seed 1); per view: forward, dL/dRGBA of sum (render - target)^2
-    (target from means + N(0, 0.01), seed 3), recompute-mode backward
-    accumulating into one 3M x 59 gradient with FP32 atomics.  Reports the
-    peak device memory in use during the timed steps."""
+    radius 25 (seed 1); per view: forward with the composited-id checkpoint
+    (one 1280x720 x 256 buffer reused by the views), dL/dRGBA of
+    sum (render - target)^2 (target from means + N(0, 0.01), seed 3), and the
+    backward accumulating into one 3M x 59 gradient with FP32 atomics.  The
+    recompute-mode step (no checkpoint; the traversal re-runs in bands) is
+    timed beside it, and replaces it if a ray composites more than 256 hits.
+    Reports the peak device memory in use during the timed steps."""
     import numpy as np
     stream = torch.cuda.current_stream()
     raw = gt.generate_raw(seed=0, config="c5")
@@ -313,12 +317,16 @@ def bench_c5(gt, torch, dev, sp, steps):
     gimg = torch.empty_like(rgba)
     grads = torch.zeros((n, gt.GT_RAW_FLOATS), dtype=torch.float32, device=dev)
     ovf = torch.zeros(1, dtype=torch.int32, device=dev)
+    cap = 256
+    cids = torch.empty((H * W, cap), dtype=torch.int32, device=dev)
+    ncomp = torch.empty(H * W, dtype=torch.int32, device=dev)
+    max_comp = torch.zeros((), dtype=torch.int32, device=dev)
 
     def fwd():
         for u in us:
             scene.render_device(u, W, H, rgba.data_ptr(), sp)
 
-    def step():
+    def step_rc():
         grads.zero_()
         for i, u in enumerate(us):
             scene.render_device(u, W, H, rgba.data_ptr(), sp)
@@ -326,22 +334,52 @@ def bench_c5(gt, torch, dev, sp, steps):
             gimg.mul_(2.0)
             gt.backward(scene, u, W, H, gimg.data_ptr(), grads.data_ptr(), None, None, 0, ovf.data_ptr(), 0, sp)
 
-    step()
-    torch.cuda.synchronize()
-    with MemSampler(torch) as mem:
-        ms = _events_ms(torch, stream, step, steps)
-    ms_f = _events_ms(torch, stream, fwd, steps)
-    assert int(ovf.item()) == 0
+    def step_ck(track_max=False):
+        grads.zero_()
+        for i, u in enumerate(us):
+            scene.render_device(u, W, H, rgba.data_ptr(), sp, comp_ids_ptr=cids.data_ptr(),
+                                n_comp_ptr=ncomp.data_ptr(), comp_cap=cap)
+            if track_max:
+                torch.maximum(max_comp, ncomp.max(), out=max_comp)
+            torch.sub(rgba, targets[i], out=gimg)
+            gimg.mul_(2.0)
+            gt.backward(scene, u, W, H, gimg.data_ptr(), grads.data_ptr(), cids.data_ptr(), ncomp.data_ptr(), cap,
+                        ovf.data_ptr(), 0, sp)
+
     rays = V * W * H
+    step_rc()
+    torch.cuda.synchronize()
+    assert int(ovf.item()) == 0
+    with MemSampler(torch) as mem_rc:
+        ms_rc = _events_ms(torch, stream, step_rc, max(1, steps // 2))
+    norm_rc = float(grads.norm().item())
+    step_ck(track_max=True)
+    torch.cuda.synchronize()
+    checkpoint = int(ovf.item()) == 0
+    if checkpoint:
+        with MemSampler(torch) as mem:
+            ms = _events_ms(torch, stream, step_ck, steps)
+        mode = f"checkpoint (cap {cap}, one buffer reused by the views)"
+    else:
+        ms, mem = ms_rc, mem_rc
+        mode = f"recompute (a ray composited more than {cap} hits)"
+    assert int(ovf.item()) == 0 or not checkpoint
+    ms_f = _events_ms(torch, stream, fwd, steps)
     info = scene.info()
     return {
         "workload": f"{n} Gaussians 256-cluster mixture, {V} cameras {W}x{H} on a sphere of radius 25, alpha_min "
-                    f"0.01, k 16, T_min 0.001; per view forward + dL/dRGBA + recompute-mode backward, gradients "
-                    f"accumulated across views",
+                    f"0.01, k 16, T_min 0.001; per view forward + dL/dRGBA + backward, gradients accumulated "
+                    f"across views",
+        "mode": mode, "max_composited_per_ray": int(max_comp.item()),
         "rays_per_step": rays, "ms_per_step": round(ms, 3), "mrays": round(rays / ms / 1e3, 3),
         "ms_forward_only": round(ms_f, 3), "ratio_vs_forward": round(ms / ms_f, 3),
         "peak_device_bytes": int(mem.peak), "scene_device_bytes": info["device_bytes"],
-        "gradient_bytes": n * gt.GT_RAW_FLOATS * 4, "grad_norm": float(grads.norm().item()), "steps": steps,
+        "checkpoint_bytes": int(cids.numel() * 4 + ncomp.numel() * 4),
+        "gradient_bytes": n * gt.GT_RAW_FLOATS * 4, "grad_norm": float(grads.norm().item()),
+        "recompute": {"ms_per_step": round(ms_rc, 3), "mrays": round(rays / ms_rc / 1e3, 3),
+                      "ratio_vs_forward": round(ms_rc / ms_f, 3), "peak_device_bytes": int(mem_rc.peak),
+                      "grad_norm": norm_rc, "steps": max(1, steps // 2)},
+        "steps": steps,
     }
 
 
