@@ -198,6 +198,29 @@ def test_backward_shared_gaussians_ragged_tiles():
     _check_grads(g, ref, "shared")
 
 
+def test_backward_block_table_overflow():
+    """20k pixel-sized Gaussians: each 16x16 block of rays composites about a
+    thousand distinct Gaussians, more than the block's 512-entry LDS table
+    holds, so many accumulators take the direct HBM atomic path."""
+    _print("case backward LDS table overflow 64x48")
+    raw = gt.generate_raw(n=20000, seed=11, config="c1", rest_std=0.2, log_scale_mean=-3.2, log_scale_std=0.3)
+    W, H = 64, 48
+    s = gt.Scene(raw, alpha_min=0.05)
+    u = uniforms([0.3, -0.2, 5.0], [0, 0, 0], W, H, hfov_deg=50, alpha_min=0.05, k=8, T_min=1e-3, sh_degree_max=3)
+    G = np.random.default_rng(11).standard_normal((H, W, 4)).astype(np.float32)
+    G[..., 3] = 0.0
+    _, ref, ck, _ = _reference(raw, s, u, W, H, G, 3)
+    ids = ck["comp_ids"].reshape(H, W, -1)
+    nc = ck["n_comp"].reshape(H, W)
+    distinct = [len({int(i) for y in range(by, by + 16) for x in range(bx, bx + 16) for i in ids[y, x, :nc[y, x]]})
+                for by in range(0, H, 16) for bx in range(0, W, 16)]
+    _print(f"  distinct Gaussians per 16x16 block: min {min(distinct)}, max {max(distinct)}")
+    assert min(distinct) > 1.5 * 512
+    g, ovf = _gpu_grads(s, raw.shape[0], u, W, H, G)
+    assert ovf == 0
+    _check_grads(g, ref, "overflow")
+
+
 def test_recompute_equals_checkpoint():
     """Recompute mode over several bands (small scratch budget) == checkpoint mode up to atomic order."""
     _print("case backward recompute vs checkpoint")
