@@ -177,3 +177,29 @@ def test_packet_traversal_debug_modes(mode):
     r = orc.render(o, orc.params_from_uniforms(u, 64, 64), max_rounds=1)
     np.testing.assert_array_equal(g["hit_ids"], r["hit_ids"])
     np.testing.assert_allclose(g["rgba"].reshape(-1, 4), r["rgba"], atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("tile_w", [0, 4, 16, 32])
+def test_host_image_equals_device_image(tile_w):
+    """A host-image render equals, byte for byte, a device-buffer render and a
+    host render that also returns counters, for every ray tile shape and
+    debug mode, at sizes that leave partial tiles."""
+    import torch
+    _print(f"case host image vs device image tile_w={tile_w}")
+    raw = gt.generate_raw(n=5000, seed=4, config="c1")
+    s = gt.Scene(raw, alpha_min=0.1)
+    s.set_tuning(tile_w=tile_w)
+    dev = torch.device("cuda", 0)
+    for (W, H) in [(1, 1), (65, 7), (37, 23), (300, 170), (1000, 563)]:
+        for mode in (0, 1, 2):
+            u = uniforms([0.3, 0.2, 5.0], [0, 0, 0], W, H, hfov_deg=40, alpha_min=0.1, k=6, T_min=0.03)
+            u.debug_mode = mode
+            banded = s.render(u, W, H)["rgba"]
+            single = s.render(u, W, H, want_stats=True)["rgba"]
+            img = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+            s.render_device(u, W, H, img.data_ptr())
+            torch.cuda.synchronize()
+            d = img.cpu().numpy()
+            assert np.array_equal(banded.view(np.uint32), single.view(np.uint32)), f"{W}x{H} mode {mode}"
+            assert np.array_equal(banded.view(np.uint32), d.view(np.uint32)), f"{W}x{H} mode {mode}"
+    s.close()
