@@ -78,6 +78,15 @@ void gt_camera_look_at(const float pos[3], const float target[3], float to_world
 
 /* ---- scene (acceleration container) ---- */
 
+/* Every scene entry point checks its records: a non-finite value, a
+ * quaternion of zero norm, or a scale exp(log-scale) that is 0 or inf in
+ * f32 (for packed records: a non-finite field, a scale <= 0 or a zero
+ * quaternion) is GT_E_INVALID, and gt_last_error() names the first such
+ * Gaussian.  The reference has no check (GS_ply.ts:225-235 divides by the
+ * quaternion norm and builds NaN proxies).  After such an error from
+ * gt_scene_update_device / gt_scene_refit_device the scene's contents are
+ * unspecified until an update with valid parameters succeeds. */
+
 /* From PLY-raw parameters (n x 59 floats): applies the activations of
  * src/GS_ply.ts:225-235 in double, packs 320-B records, filters proxies by
  * opacity > alpha_min and builds their transforms as src/main.ts:73-94 does,
@@ -290,8 +299,10 @@ int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, const uint32_
                     const gt_frame* outs, gt_batch** out);
 /* Renders every camera with one launch on stream (asynchronous).  With
  * use_graph != 0 the launches are captured into a hipGraph on first use and
- * the graph is replayed afterwards (recaptured if the scene's tuning
- * changes).  Each camera's results equal gt_render on that camera. */
+ * the graph is replayed afterwards; it is recaptured when the scene's tuning
+ * changes or a rebuild has moved its device arrays (gt_scene_set_alpha_min,
+ * gt_scene_update_device, a refit that rebuilds).  Each camera's results
+ * equal gt_render on that camera. */
 int gt_batch_render(gt_batch* b, int use_graph, void* hip_stream);
 int gt_batch_rays(const gt_batch* b, uint64_t* rays);
 void gt_batch_destroy(gt_batch* b);

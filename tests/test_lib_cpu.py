@@ -148,7 +148,10 @@ def test_ply_reader_errors(tmp_path):
     with pytest.raises(gt.GTError):
         gt.read_ply(str(p))
     p2 = tmp_path / "trunc.ply"
-    p2.write_bytes(b"ply\nformat binary_little_endian 1.0\nelement vertex 10\nproperty float x\nend_header\n" + b"\0" * 8)
+    p2.write_bytes(b"ply\nformat binary_little_endian 1.0\nelement vertex 10\n" + b"".join(
+        b"property float %s\n" % n for n in [b"x", b"y", b"z", b"f_dc_0", b"f_dc_1", b"f_dc_2", b"opacity",
+                                              b"scale_0", b"scale_1", b"scale_2", b"rot_0", b"rot_1", b"rot_2",
+                                              b"rot_3"]) + b"end_header\n" + b"\0" * 8)
     with pytest.raises(gt.GTError):
         gt.read_ply(str(p2))
     # degree-0 file: uchar colour-free, only f_dc; reads with zero f_rest

@@ -84,9 +84,16 @@ extern "C" int gt_autotune_frame(gt_scene* s, const gt_uniforms* u, uint32_t wid
   return tune(s, iters, st, out, [&]() { return gt_render(s, u, width, height, &f, GT_BUF_DEVICE, stream); });
 }
 
+gt_scene* gt_batch_scene(const gt_batch* b); /* gt_batch.cpp */
+
 extern "C" int gt_autotune_batch(gt_scene* s, gt_batch* b, int use_graph, uint32_t iters, void* stream,
                                  gt_autotune_result* out) {
   if (!s || !b || !out) { gt_set_error("gt_autotune_batch: NULL argument"); return GT_E_INVALID; }
+  if (gt_batch_scene(b) != s) {
+    /* the candidates are set on s but the batch renders its own scene */
+    gt_set_error("gt_autotune_batch: the batch renders another scene than the one being tuned");
+    return GT_E_INVALID;
+  }
   hipStream_t st = (hipStream_t)stream;
   return tune(s, iters, st, out, [&]() { return gt_batch_render(b, use_graph, stream); });
 }

@@ -39,6 +39,13 @@ struct gt_batch {
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  /* what the captured launch took by value: the scene's device arrays and
+   * counts, and the persistent renderer's work counter.  A rebuild may move
+   * or resize the arrays (gt_scene_set_alpha_min, a GPU rebuild or a refit
+   * that falls back to one), after which the graph would read freed memory;
+   * the graph is recaptured whenever these differ from the scene's. */
+  gt_dev_scene dev_used{};
+  unsigned int* work_used = nullptr;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -53,9 +60,13 @@ struct gt_batch {
   }
 };
 
-/* (re)numbers the tiles for the scene's current tile shape and uploads the table */
+/* (re)numbers the tiles for the scene's current tile shape and uploads the
+ * table; drops a captured graph whose launch arguments are out of date */
 static int prepare_table(gt_batch* b) {
   const gt_tuning* t = gt_scene_tuning(b->scene);
+  if (b->exec && (std::memcmp(gt_scene_dev(b->scene), &b->dev_used, sizeof(gt_dev_scene)) != 0 ||
+                  gt_scene_work(b->scene) != b->work_used))
+    b->drop_graph();
   if (b->table_valid && std::memcmp(t, &b->tune_used, sizeof(gt_tuning)) == 0) return GT_OK;
   uint64_t base = 0;
   for (auto& f : b->frames) {
@@ -106,6 +117,9 @@ extern "C" int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, co
     gt_set_error("gt_batch_create: scene, uniforms, sizes and outputs are required (n >= 1)");
     return GT_E_INVALID;
   }
+  /* the constant face table is uploaded here, never inside a graph capture */
+  int prc = gt_forward_prepare();
+  if (prc != GT_OK) return prc;
   gt_batch* b = new (std::nothrow) gt_batch();
   if (!b) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
   b->scene = s;
@@ -187,6 +201,8 @@ extern "C" int gt_batch_render(gt_batch* b, int use_graph, void* stream) {
       gt_set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
       return GT_E_HIP;
     }
+    b->dev_used = *gt_scene_dev(b->scene);
+    b->work_used = gt_scene_work(b->scene);
   }
   hipError_t e = hipGraphLaunch(b->exec, st);
   if (e != hipSuccess) {
@@ -195,6 +211,8 @@ extern "C" int gt_batch_render(gt_batch* b, int use_graph, void* stream) {
   }
   return GT_OK;
 }
+
+gt_scene* gt_batch_scene(const gt_batch* b) { return b->scene; }
 
 extern "C" int gt_batch_rays(const gt_batch* b, uint64_t* rays) {
   if (!b || !rays) { gt_set_error("gt_batch_rays: NULL argument"); return GT_E_INVALID; }

@@ -72,7 +72,7 @@
 
 /* counters (u32); CTL_LVL + L = fronts (4-wide nodes) of collapse level L */
 enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CTL_ITEMS = 10, CTL_STALE = 11,
-       CTL_PNODES = 12, CTL_LVL = 16,
+       CTL_PNODES = 12, CTL_BAD = 13 /* first invalid Gaussian (gt_prep_valid), ~0 if none */, CTL_LVL = 16,
        CTL_WORDS = CTL_LVL + GT_STACK_MAX + 4 };
 #define COLLAPSE_BATCH 8 /* collapse levels launched between host reads of the level counts */
 #define MID_LEAVES (16 * BUILD_BLOCK) /* leaves per refit phase-2 workgroup (LDS counters) */
@@ -181,6 +181,7 @@ __global__ void k_init(unsigned int* ctl) {
   for (uint32_t i = threadIdx.x; i < CTL_WORDS; i += blockDim.x) {
     uint32_t v = 0;
     if (i >= CTL_CMIN && i < CTL_CMIN + 3) v = 0xffffffffu;
+    if (i == CTL_BAD) v = 0xffffffffu;
     if (i == CTL_NODES) v = 1u;   /* node 0 = root */
     if (i == CTL_LVL) v = 1u;     /* level 0 = the root's front */
     ctl[i] = v;
@@ -196,7 +197,8 @@ __global__ void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
 __global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ raw, uint32_t n, double alpha_min,
                                                       float* __restrict__ geom, float* __restrict__ sh,
                                                       float* __restrict__ o2w, float* __restrict__ w2o,
-                                                      float4* __restrict__ box, uint32_t* __restrict__ part) {
+                                                      float4* __restrict__ box, uint32_t* __restrict__ part,
+                                                      unsigned int* __restrict__ bad) {
   const uint32_t i = blockIdx.x * BUILD_BLOCK + threadIdx.x;
   bool has = false;
   float c[3] = {0.0f, 0.0f, 0.0f};
@@ -208,6 +210,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ 
     double act[8];
     float g[12], s[48];
     gt_prep_activate(r, act, g, s);
+    if (!gt_prep_valid(r, act)) atomicMin(bad, i);
     float4* G = reinterpret_cast<float4*>(geom) + 3 * (size_t)i;
     G[0] = make_float4(g[0], g[1], g[2], g[3]);
     G[1] = make_float4(g[4], g[5], g[6], g[7]);
@@ -1021,6 +1024,14 @@ static int read_ctl(gt_devbuild* w, hipStream_t st) {
   return GT_OK;
 }
 
+/* GT_E_INVALID naming the first Gaussian k_prep found invalid (gt_prep_valid) */
+static int check_bad(const gt_devbuild* w) {
+  const uint32_t bad = w->host_ctl[CTL_BAD];
+  if (bad == 0xffffffffu) return GT_OK;
+  gt_set_error("Gaussian %u: non-finite parameter, zero quaternion or a scale that is 0 or inf in f32", bad);
+  return GT_E_INVALID;
+}
+
 int gt_device_build(gt_scene* s, hipStream_t st) {
   const auto t0 = std::chrono::steady_clock::now();
   if (!s->db) {
@@ -1046,7 +1057,7 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
   if (n > 0) {
     k_prep<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float*)s->raw_dev.p, n, s->alpha_min, (float*)s->geom.p,
                                                   (float*)s->sh.p, (float*)w->o2w.p, (float*)w->w2o.p,
-                                                  (float4*)w->box.p, (uint32_t*)w->part.p);
+                                                  (float4*)w->box.p, (uint32_t*)w->part.p, ctl + CTL_BAD);
     HIP_TRY(hipGetLastError());
     k_bounds<<<1, 1024, 0, st>>>((const uint32_t*)w->part.p, blocks_for(n), ctl);
     HIP_TRY(hipGetLastError());
@@ -1061,6 +1072,7 @@ int gt_device_build(gt_scene* s, hipStream_t st) {
                                      (unsigned long long*)w->keys_b.p, n, 0, 64, st));
   }
   GT_TRY(read_ctl(w, st));
+  GT_TRY(check_bad(w));
   const uint32_t np = w->host_ctl[CTL_NPROX];
   if (np > n || np > GT_MAX_PROXIES) {
     gt_set_error("device build: %u proxies for %u Gaussians (limit %u)", np, n, GT_MAX_PROXIES);
@@ -1271,7 +1283,7 @@ int gt_device_refit(gt_scene* s, const float* raw_dev, hipStream_t st, uint32_t*
   if (n > 0) {
     k_prep<<<blocks_for(n), BUILD_BLOCK, 0, st>>>((const float*)s->raw_dev.p, n, s->alpha_min, (float*)s->geom.p,
                                                   (float*)s->sh.p, (float*)w->o2w.p, (float*)w->w2o.p,
-                                                  (float4*)w->box.p, (uint32_t*)w->part.p);
+                                                  (float4*)w->box.p, (uint32_t*)w->part.p, ctl + CTL_BAD);
     HIP_TRY(hipGetLastError());
     k_bounds<<<1, 1024, 0, st>>>((const uint32_t*)w->part.p, blocks_for(n), ctl);
     HIP_TRY(hipGetLastError());
@@ -1291,6 +1303,7 @@ int gt_device_refit(gt_scene* s, const float* raw_dev, hipStream_t st, uint32_t*
     }
   }
   GT_TRY(read_ctl(w, st));
+  GT_TRY(check_bad(w));
   s->device_built = true;
   s->h = HostScene{};
   if (w->host_ctl[CTL_STALE]) {

@@ -943,6 +943,15 @@ hipError_t ensure_faces() {
 
 }  // namespace
 
+extern "C" int gt_forward_prepare(void) {
+  const hipError_t e = ensure_faces();
+  if (e != hipSuccess) {
+    gt_set_error("face table upload failed: %s", hipGetErrorString(e));
+    return GT_E_HIP;
+  }
+  return GT_OK;
+}
+
 static void tuning_defaults(const gt_tuning* t, uint32_t* tile_w, uint32_t* lds_depth) {
   *tile_w = (t && t->tile_w) ? t->tile_w : 8u;
   *lds_depth = (t && t->stack_lds) ? t->stack_lds : (uint32_t)GT_LDS_STACK;

@@ -112,6 +112,9 @@ int gt_launch_persistent(const gt_dev_scene* s, const gt_dev_frame* cam0, const 
 /* out[i] (float4, N entries) = FP32 radiance of Gaussian i seen from f's
  * camera origin, bit-identical to the colour the render kernels composite. */
 int gt_launch_colors(const gt_dev_scene* s, const gt_dev_frame* f, float* out, void* stream);
+/* Uploads the icosahedron face table to the current device once (a
+ * synchronous copy: call it outside any stream capture); GT_OK or GT_E_HIP. */
+int gt_forward_prepare(void);
 /* avg[i] = gt_aa_mix(avg[i], img[i], frames) per RGB channel, A = 1 (float4 pixels) */
 int gt_launch_aa_mix(const float* img, float* avg, uint32_t n_pixels, uint32_t frames, void* stream);
 void gt_set_error(const char* fmt, ...);

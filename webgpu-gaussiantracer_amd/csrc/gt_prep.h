@@ -58,6 +58,23 @@ GT_HD void gt_prep_activate(const float* r, double* act, float* geom, float* sh)
     for (int c = 0; c < 3; ++c) sh[3 * (1 + j) + c] = r[6 + c * 15 + j];
 }
 
+/* Whether PLY-raw record r, with its activations act (gt_prep_activate),
+ * describes a finite Gaussian: every raw value finite, a quaternion of
+ * nonzero norm (else q / |q| is 0/0) and scales exp(log-scale) finite and
+ * nonzero in f32 (else the proxy transform is singular or infinite).  The
+ * reference has no such check (GS_ply.ts:225-235 divides by a zero norm and
+ * builds NaN proxies); here such a record is an argument error. */
+GT_HD int gt_prep_valid(const float* r, const double* act) {
+  int ok = 1;
+  for (int j = 0; j < GT_RAW_FLOATS; ++j) ok &= __builtin_isfinite(r[j]) ? 1 : 0;
+  for (int j = 1; j < 4; ++j) {
+    const float sc = (float)act[j];
+    ok &= (sc > 0.0f && __builtin_isfinite(sc)) ? 1 : 0;
+  }
+  for (int j = 4; j < 8; ++j) ok &= __builtin_isfinite(act[j]) ? 1 : 0;
+  return ok;
+}
+
 /* Rotation of wgpu-matrix mat3.fromQuat([x, y, z, w]) evaluated in double
  * and stored as f32; R[3c + r] = column c, row r. */
 GT_HD void gt_prep_quat_to_mat(double x, double y, double z, double w, float* R) {

@@ -57,33 +57,61 @@ static void parallel_chunks(size_t n, unsigned chunks, F f) {
   for (auto& x : th) x.join();
 }
 
-static void activate_raw(HostScene& h) {
+/* Activations of every raw record; returns GT_OK, or GT_E_INVALID naming
+ * the first record that gt_prep_valid rejects. */
+static int activate_raw(HostScene& h) {
   h.act.assign((size_t)h.n * 8, 0.0);
   h.packed.assign((size_t)h.n * 80, 0.0f);
-  parallel_chunks(h.n, host_threads(h.n), [&](size_t i0, size_t i1, unsigned) {
+  const unsigned chunks = host_threads(h.n);
+  std::vector<uint32_t> bad(chunks, 0xffffffffu);
+  parallel_chunks(h.n, chunks, [&](size_t i0, size_t i1, unsigned c) {
   for (size_t i = i0; i < i1; ++i) {
     float g[12], sh[48];
     gt_prep_activate(&h.raw[(size_t)i * GT_RAW_FLOATS], &h.act[(size_t)i * 8], g, sh);
+    if (bad[c] == 0xffffffffu && !gt_prep_valid(&h.raw[(size_t)i * GT_RAW_FLOATS], &h.act[(size_t)i * 8]))
+      bad[c] = (uint32_t)i;
     float* o = &h.packed[(size_t)i * 80];
     o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
     o[4] = g[4]; o[5] = g[5]; o[6] = g[6];
     o[8] = g[8]; o[9] = g[9]; o[10] = g[10]; o[11] = g[11];
     o[12] = g[3];
     for (int j = 0; j < 16; ++j)
-      for (int c = 0; c < 3; ++c) o[16 + 4 * j + c] = sh[3 * j + c];
+      for (int cc = 0; cc < 3; ++cc) o[16 + 4 * j + cc] = sh[3 * j + cc];
   }
   });
+  for (unsigned c = 0; c < chunks; ++c)
+    if (bad[c] != 0xffffffffu) {
+      gt_set_error("Gaussian %u: non-finite parameter, zero quaternion or a scale that is 0 or inf in f32", bad[c]);
+      return GT_E_INVALID;
+    }
+  return GT_OK;
 }
 
-static void activate_packed(HostScene& h) {
+/* The activated values of packed 320-B records in double (the JS doubles
+ * are not part of the buffer).  A record must be finite in every field the
+ * path reads (position, scale, quaternion, opacity, the 16 SH vec3), with
+ * scales > 0 and a nonzero quaternion; GT_E_INVALID names the first that
+ * is not. */
+static int activate_packed(HostScene& h) {
   h.act.assign((size_t)h.n * 8, 0.0);
   for (uint32_t i = 0; i < h.n; ++i) {
     const float* o = &h.packed[(size_t)i * 80];
     double* a = &h.act[(size_t)i * 8];
+    bool ok = std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]) && std::isfinite(o[12]);
+    for (int j = 0; j < 3; ++j) ok = ok && std::isfinite(o[4 + j]) && o[4 + j] > 0.0f;
+    for (int j = 0; j < 4; ++j) ok = ok && std::isfinite(o[8 + j]);
+    ok = ok && (o[8] != 0.0f || o[9] != 0.0f || o[10] != 0.0f || o[11] != 0.0f);
+    for (int j = 0; j < 16; ++j)
+      for (int c = 0; c < 3; ++c) ok = ok && std::isfinite(o[16 + 4 * j + c]);
+    if (!ok) {
+      gt_set_error("Gaussian %u: non-finite field, scale <= 0 or zero quaternion in the packed record", i);
+      return GT_E_INVALID;
+    }
     a[0] = o[12];
     for (int j = 0; j < 3; ++j) a[1 + j] = o[4 + j];
     for (int j = 0; j < 4; ++j) a[4 + j] = o[8 + j];
   }
+  return GT_OK;
 }
 
 /* ---------------- proxies (src/main.ts:73-94, webrtx/bvh/src/lib.rs:501-505) ---------------- */
@@ -580,6 +608,9 @@ int gt_devbuf_reserve(DevBuf& b, size_t bytes) {
 
 int gt_scene_static_upload(gt_scene* s) {
   int rc;
+  /* the render kernels' constant face table, before any render (which may
+   * be captured into a graph) needs it */
+  if ((rc = gt_forward_prepare()) != GT_OK) return rc;
   if (!s->faces.p) {
     float faces[12 * GT_NUM_FACES];
     face_table(faces);
@@ -698,7 +729,8 @@ extern "C" int gt_scene_create_raw(const float* raw, uint32_t n, double alpha_mi
     s->n = n;
     s->h.raw.assign(raw, raw + (size_t)n * GT_RAW_FLOATS);
     s->alpha_min = alpha_min;
-    activate_raw(s->h);
+    const int rc = activate_raw(s->h);
+    if (rc != GT_OK) return rc;
     return finish_create(s, out);
   } catch (const std::bad_alloc&) {
     gt_set_error("out of host memory");
@@ -722,7 +754,8 @@ extern "C" int gt_scene_create_packed(const void* records, uint32_t n, double al
     const float* f = (const float*)records;
     s->h.packed.assign(f, f + (size_t)n * 80);
     s->alpha_min = alpha_min;
-    activate_packed(s->h);
+    const int rc = activate_packed(s->h);
+    if (rc != GT_OK) return rc;
     return finish_create(s, out);
   } catch (const std::bad_alloc&) {
     gt_set_error("out of host memory");
@@ -1108,7 +1141,8 @@ extern "C" int gt_host_prepare(const float* raw, uint32_t n, double alpha_min, f
     HostScene h;
     h.n = n;
     h.raw.assign(raw, raw + (size_t)n * GT_RAW_FLOATS);
-    activate_raw(h);
+    const int rc = activate_raw(h);
+    if (rc != GT_OK) return rc;
     build_proxies(h, alpha_min);
     if (packed) std::memcpy(packed, h.packed.data(), h.packed.size() * sizeof(float));
     if (proxy_ids) std::memcpy(proxy_ids, h.proxy_ids.data(), h.proxy_ids.size() * sizeof(uint32_t));

@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
+#include <stdexcept>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -138,11 +140,28 @@ extern "C" int gt_generate_scene(const gt_gen_params* g, float* raw) {
  * uchar properties as value/255 (GS_ply.ts:122-128).  Unlike the reference,
  * other property types are sized correctly instead of being skipped without
  * advancing (GS_ply.ts:120-129 mis-parses them), and SH degree < 3 leaves the
- * missing coefficients at zero instead of breaking the 320-B layout. */
-extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, uint32_t* deg_out) {
-  if (!path || !raw_out || !n_out) { gt_set_error("gt_ply_read: NULL argument"); return GT_E_INVALID; }
-  *raw_out = nullptr;
-  *n_out = 0;
+ * missing coefficients at zero instead of breaking the 320-B layout.
+ * The file is untrusted input: the header's vertex count is checked against
+ * the bytes the file holds before anything is allocated; every property
+ * index must lie in its field (f_dc 0..2, scale 0..2, rot 0..3, f_rest below
+ * the f_rest count, which must be a multiple of 3); the 14 fields besides
+ * f_rest that the reference reads (GS_ply.ts:225-235) must be present; and
+ * no exception leaves the call. */
+namespace {
+
+/* decimal index after a property-name prefix; -1 unless the rest is 1-6 digits */
+int prop_index(const std::string& name, size_t prefix) {
+  if (name.size() <= prefix || name.size() - prefix > 6) return -1;
+  int v = 0;
+  for (size_t i = prefix; i < name.size(); ++i) {
+    const char c = name[i];
+    if (c < '0' || c > '9') return -1;
+    v = v * 10 + (c - '0');
+  }
+  return v;
+}
+
+int ply_read(const char* path, float** raw_out, uint32_t* n_out, uint32_t* deg_out) {
   std::ifstream f(path, std::ios::binary);
   if (!f) { gt_set_error("cannot open %s", path); return GT_E_IO; }
   std::string line;
@@ -151,8 +170,10 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
   struct Prop { std::string name, type; int size; };
   std::vector<Prop> props;
   uint64_t n = 0;
-  bool in_vertex = false, binary_le = false, header_done = false;
+  bool in_vertex = false, binary_le = false, header_done = false, seen_vertex = false;
+  int header_lines = 0;
   while (std::getline(f, line)) {
+    if (++header_lines > 100000) { gt_set_error("%s: header too long", path); return GT_E_IO; }
     if (!line.empty() && line.back() == '\r') line.pop_back();
     std::istringstream ls(line);
     std::string w;
@@ -165,7 +186,25 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
       std::string name;
       ls >> name;
       in_vertex = name == "vertex";
-      if (in_vertex) ls >> n;
+      if (in_vertex) {
+        if (seen_vertex) { gt_set_error("%s: two vertex elements", path); return GT_E_IO; }
+        seen_vertex = true;
+        std::string cnt;
+        ls >> cnt;
+        if (cnt.empty() || cnt.size() > 19 || cnt.find_first_not_of("0123456789") != std::string::npos) {
+          gt_set_error("%s: bad vertex count '%s'", path, cnt.c_str());
+          return GT_E_IO;
+        }
+        n = std::strtoull(cnt.c_str(), nullptr, 10);
+      } else if (!seen_vertex) {
+        /* its records would precede the vertex records in the body */
+        std::string cnt;
+        ls >> cnt;
+        if (cnt != "0") {
+          gt_set_error("%s: element %s before the vertex element is not supported", path, name.c_str());
+          return GT_E_IO;
+        }
+      }
     } else if (w == "property" && in_vertex) {
       std::string type, name;
       ls >> type >> name;
@@ -183,6 +222,8 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
         gt_set_error("%s: unsupported property type %s", path, type.c_str());
         return GT_E_IO;
       }
+      for (const Prop& q : props)
+        if (q.name == name) { gt_set_error("%s: property %s given twice", path, name.c_str()); return GT_E_IO; }
       props.push_back({name, type, size});
     } else if (w == "end_header") {
       header_done = true;
@@ -196,6 +237,10 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
   int n_rest = 0;
   for (auto& p : props)
     if (p.name.rfind("f_rest_", 0) == 0) n_rest++;
+  if (n_rest % 3 != 0) {
+    gt_set_error("%s: %d f_rest coefficients is not a multiple of 3", path, n_rest);
+    return GT_E_IO;
+  }
   const int per_color = n_rest / 3;
   int deg = 0;
   while ((deg + 1) * (deg + 1) - 1 < per_color) deg++;
@@ -203,24 +248,51 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
     gt_set_error("%s: %d f_rest coefficients is not SH degree 0..3", path, n_rest);
     return GT_E_IO;
   }
-  // column of each raw field (-1 = absent)
+  // column of each raw field (-1 = a property the reader ignores)
   std::vector<int> col(props.size(), -1);
+  uint32_t present = 0; /* bit b: raw column b (0..5) or 45 + b (51..58) was found */
   for (size_t i = 0; i < props.size(); ++i) {
     const std::string& nm = props[i].name;
     int idx = -1;
+    const bool dc = nm.rfind("f_dc_", 0) == 0, rest = nm.rfind("f_rest_", 0) == 0;
+    const bool scale = nm.rfind("scale_", 0) == 0, rot = nm.rfind("rot_", 0) == 0;
     if (nm == "x") idx = 0; else if (nm == "y") idx = 1; else if (nm == "z") idx = 2;
     else if (nm == "opacity") idx = 51;
-    else if (nm.rfind("f_dc_", 0) == 0) idx = 3 + std::atoi(nm.c_str() + 5);
-    else if (nm.rfind("scale_", 0) == 0) idx = 52 + std::atoi(nm.c_str() + 6);
-    else if (nm.rfind("rot_", 0) == 0) idx = 55 + std::atoi(nm.c_str() + 4);
-    else if (nm.rfind("f_rest_", 0) == 0) {
-      // f_rest_{rgb*per_color + j} -> raw f_rest_{rgb*15 + j}
-      const int r = std::atoi(nm.c_str() + 7);
-      if (per_color > 0) idx = 6 + (r / per_color) * 15 + (r % per_color);
+    else if (dc || rest || scale || rot) {
+      const size_t pre = dc ? 5 : rest ? 7 : scale ? 6 : 4;
+      const int lim = dc ? 3 : rest ? n_rest : scale ? 3 : 4;
+      const int r = prop_index(nm, pre);
+      if (r < 0 || r >= lim) {
+        gt_set_error("%s: property %s names no field (index out of range)", path, nm.c_str());
+        return GT_E_IO;
+      }
+      if (dc) idx = 3 + r;
+      else if (rest) idx = 6 + (r / per_color) * 15 + (r % per_color); /* f_rest_{rgb*per_color+j} -> rgb*15+j */
+      else if (scale) idx = 52 + r;
+      else idx = 55 + r;
     }
     col[i] = idx;
+    if (idx >= 0 && idx < 6) present |= 1u << idx;
+    if (idx >= 51) present |= 1u << (idx - 45);
   }
-  std::vector<char> buf(stride * (size_t)n);
+  if (present != (1u << 14) - 1u) {
+    gt_set_error("%s: x y z, f_dc_0..2, opacity, scale_0..2 and rot_0..3 are required", path);
+    return GT_E_IO;
+  }
+  /* the records must be in the file before anything is sized by n */
+  const std::streamoff data_pos = f.tellg();
+  f.seekg(0, std::ios::end);
+  const std::streamoff end_pos = f.tellg();
+  if (data_pos < 0 || end_pos < data_pos) { gt_set_error("%s: cannot size the vertex data", path); return GT_E_IO; }
+  const uint64_t avail = (uint64_t)(end_pos - data_pos);
+  uint64_t need = 0;
+  if (__builtin_mul_overflow((uint64_t)stride, n, &need) || need > avail) {
+    gt_set_error("%s: truncated vertex data (%llu vertices of %zu B, %llu B in the file)", path,
+                 (unsigned long long)n, stride, (unsigned long long)avail);
+    return GT_E_IO;
+  }
+  f.seekg(data_pos);
+  std::vector<char> buf((size_t)need);
   f.read(buf.data(), (std::streamsize)buf.size());
   if ((size_t)f.gcount() != buf.size()) { gt_set_error("%s: truncated vertex data", path); return GT_E_IO; }
   float* raw = (float*)std::calloc((size_t)n * GT_RAW_FLOATS + 1, sizeof(float));
@@ -239,7 +311,7 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
       else if (pr.type == "uint" || pr.type == "uint32") { uint32_t x; std::memcpy(&x, p, 4); val = x; }
       else if (pr.type == "short" || pr.type == "int16") { int16_t x; std::memcpy(&x, p, 2); val = x; }
       else { uint16_t x; std::memcpy(&x, p, 2); val = x; }
-      if (col[i] >= 0 && col[i] < GT_RAW_FLOATS) r[col[i]] = (float)val;
+      if (col[i] >= 0) r[col[i]] = (float)val;
       p += pr.size;
     }
   }
@@ -247,4 +319,21 @@ extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, u
   *n_out = (uint32_t)n;
   if (deg_out) *deg_out = (uint32_t)deg;
   return GT_OK;
+}
+
+}  // namespace
+
+extern "C" int gt_ply_read(const char* path, float** raw_out, uint32_t* n_out, uint32_t* deg_out) {
+  if (!path || !raw_out || !n_out) { gt_set_error("gt_ply_read: NULL argument"); return GT_E_INVALID; }
+  *raw_out = nullptr;
+  *n_out = 0;
+  try {
+    return ply_read(path, raw_out, n_out, deg_out);
+  } catch (const std::bad_alloc&) {
+    gt_set_error("%s: out of host memory", path);
+    return GT_E_NOMEM;
+  } catch (const std::exception& e) {
+    gt_set_error("%s: %s", path, e.what());
+    return GT_E_IO;
+  }
 }

@@ -154,7 +154,7 @@ def lib():
 
 def _check(rc):
     if rc != GT_OK:
-        raise GTError(rc, _lib.gt_last_error().decode())
+        raise GTError(rc, _lib.gt_last_error().decode(errors="replace"))
 
 
 def _ptr(a):
@@ -162,7 +162,7 @@ def _ptr(a):
 
 
 def last_error():
-    return _lib.gt_last_error().decode()
+    return _lib.gt_last_error().decode(errors="replace")
 
 
 def camera_look_at(pos, target):
