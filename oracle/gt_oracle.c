@@ -377,6 +377,7 @@ static void test_proxy(ray_ctx* r, const gto_scene* s, uint32_t pi) {
     if (!gt_tri_hit(oo, od, r->tmin, r->tmax, fc, &t)) continue;
     int front = gt_front_facing(w2o, fc.n, r->d);
     if (g_mutation == 4) front = !front;
+    if (g_mutation == 11 && r->count_only) front = 1; /* debugMode 2 counting back faces too */
     if (!r->backfaces && !front) continue;
     r->hits++;
     if (r->count_only) { r->count++; continue; }
@@ -459,10 +460,17 @@ static void render_ray(render_job* J, uint32_t li, uint64_t* st) {
     r.k = 1;
     r.kc[0].t = p->t_max; r.kc[0].id = GT_SENTINEL;
     r.tmin = p->t_min;
-    r.backfaces = 1;
+    r.backfaces = g_mutation == 8 ? 0 : 1; /* mutation 8: culling back faces as mode 0 does */
     gather(&r, s, J->b);
     if (r.kc[0].id != GT_SENTINEL) {
-      gto_radiance(s->packed + (size_t)GTO_PACKED_FLOATS * r.kc[0].id, p->sh_degree_max, d3, L);
+      const float* rec = s->packed + (size_t)GTO_PACKED_FLOATS * r.kc[0].id;
+      if (g_mutation == 9) { /* the commented-out alternative, GS_shader.ts:320 */
+        gt_f3 dir = gt_normalize(gt_sub(gt_f3_make(rec[0], rec[1], rec[2]), r.o));
+        float dd[3] = {dir.x, dir.y, dir.z};
+        gto_radiance(rec, p->sh_degree_max, dd, L);
+      } else {
+        gto_radiance(rec, p->sh_degree_max, d3, L);
+      }
     }
     rounds = 1;
     if (ids_out) ids_out[0] = r.kc[0].id;
@@ -477,7 +485,7 @@ static void render_ray(render_job* J, uint32_t li, uint64_t* st) {
     uint64_t c = r.count;
     float col[3];
     if (c == 0) { col[0] = 0; col[1] = 0; col[2] = 0; }
-    else if (c < 20) { col[0] = 0.5f; col[1] = 0.5f; col[2] = 0; }
+    else if (c < 20 || (g_mutation == 10 && c == 20)) { col[0] = 0.5f; col[1] = 0.5f; col[2] = 0; }
     else if (c < 40) { col[0] = 0.5f; col[1] = 0; col[2] = 0.5f; }
     else if (c < 60) { col[0] = 0; col[1] = 0.5f; col[2] = 0.5f; }
     else if (c < 80) { col[0] = 0.5f; col[1] = 0; col[2] = 0; }

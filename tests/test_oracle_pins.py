@@ -448,12 +448,246 @@ def check_invariants_and_reeval():
     assert np.abs(r["T"] - T64).max() < 1e-5
 
 
+def _fp64_hits(raw, alpha, u, W, H):
+    """Every proxy face the line of each camera ray crosses in (t_min, t_max),
+    by FP64 Moller-Trumbore on the double inverse of each proxy transform:
+    per ray (hits sorted by t as (t, Gaussian id, front), ambiguous flag).
+    Front means n . (W2O d) < 0 with the outward normal n = cross(e1, e0)
+    (the facing rule of trace.glsl:153-164 written with n_w . d = n . W2O d).
+    A ray is ambiguous when a crossing lies within 1e-5 of a face edge, of
+    t_min or of grazing incidence, or two crossings lie within 1e-5 in t."""
+    packed, act = orc.activate(raw)
+    ids, o2w, _ = orc.build_proxies(raw, act, alpha)
+    V, I = ico_geometry()
+    P0, P1, P2 = V[I[:, 0]], V[I[:, 1]], V[I[:, 2]]
+    E1, E2 = P1 - P0, P2 - P0
+    Nrm = np.cross(P0 - P2, P1 - P0)
+    p = orc.params_from_uniforms(u, W, H)
+    o = np.array(u.to_world[12:15], np.float64)
+    A = o2w.astype(np.float64).reshape(-1, 4, 3).transpose(0, 2, 1)
+    Ainv = np.linalg.inv(A[:, :, :3])
+    oo = np.einsum("nij,nj->ni", Ainv, o[None, :] - A[:, :, 3])
+    out = []
+    for py in range(H):
+        for px in range(W):
+            d = orc.ray_dir(p, px, py).astype(np.float64)
+            od = Ainv @ d
+            with np.errstate(divide="ignore", invalid="ignore"):
+                h = np.cross(od[:, None, :], E2[None, :, :])
+                f = 1.0 / (E1[None] * h).sum(-1)
+                sv = oo[:, None, :] - P0[None]
+                uu = f * (sv * h).sum(-1)
+                q = np.cross(sv, E1[None])
+                vv = f * (od[:, None, :] * q).sum(-1)
+                t = f * (E2[None] * q).sum(-1)
+            m = np.minimum(np.minimum(uu, vv), 1 - uu - vv)
+            ndot = (Nrm[None] * od[:, None, :]).sum(-1)
+            ok = np.isfinite(t) & np.isfinite(m)
+            near_edge = ok & (np.abs(m) < 1e-5 * (1 + np.abs(t))) & (t > u.t_min - 1e-4)
+            inside = ok & (m > 0) & (t > u.t_min) & (t < u.t_max)
+            amb = bool(near_edge.any()) or bool((inside & (np.abs(ndot) < 1e-6)).any()) or \
+                bool((ok & (m > 0) & (np.abs(t - u.t_min) < 1e-5)).any())
+            jj, ff = np.nonzero(inside)
+            hits = sorted((float(t[j, k]), int(ids[j]), bool(ndot[j, k] < 0)) for j, k in zip(jj, ff))
+            ts = [x[0] for x in hits]
+            if any(abs(a - b) < 1e-5 * (1 + abs(a)) for a, b in zip(ts, ts[1:])):
+                amb = True
+            out.append((hits, amb))
+    return out
+
+
+def _radiance_real_sh(rec, deg, d):
+    """max(0, 0.5 + sum_lm Y_lm(d) sh_lm) with scipy's real SH (the SH pin's basis)."""
+    dd = np.asarray(d, np.float64)
+    basis = [_real_sh(l, m, dd) for l in range(deg + 1) for m in range(-l, l + 1)]
+    sh = np.array([rec[16 + 4 * i:16 + 4 * i + 3] for i in range(len(basis))], np.float64)
+    return np.maximum(0.0, 0.5 + np.array(basis) @ sh)
+
+
+def _debug_scenes():
+    """A small random scene seen from outside and from inside its volume."""
+    raw = gt.generate_raw(n=120, seed=14, config="c1", box_half=0.6, log_scale_mean=-1.8)
+    return raw, [([0.0, 0.0, 3.0], [0.0, 0.0, 0.0]), ([0.05, -0.1, 0.15], [0.3, 0.2, -2.0])]
+
+
+def check_debug1_back_face():
+    """debugMode 1 traces with no cull flag (GS_shader.ts:292-294): a camera
+    inside a proxy sees that Gaussian through its exit (back) face, which
+    debugMode 0 culls; the colour is the SH radiance along the ray direction
+    (GS_shader.ts:318-319), not along normalize(mu - o)."""
+    ga = ([0.0, 0.0, 0.0], [-0.5] * 3, QA, 3.0, [0.8, -0.3, 0.1], np.linspace(-0.3, 0.3, 45))
+    gb = ([0.0, 0.0, -6.0], [-1.0] * 3, QB, 2.0, [-0.2, 0.6, 0.4], np.linspace(0.2, -0.2, 45))
+    raw = np.concatenate([packed_record(*ga), packed_record(*gb)])
+    s = orc.OracleScene(raw, 0.01)
+    tw = gt.camera_look_at([0.1, 0.2, 0.3], [0.4, -0.3, -5.0])
+    for mode in (0, 1):
+        u = gt.make_uniforms(tw, 1, 1, hfov_deg=30, alpha_min=0.01, k=4, T_min=1e-3, debug_mode=mode)
+        p = orc.params_from_uniforms(u, 1, 1)
+        r = orc.render(s, p, max_rounds=4, use_bvh=False)
+        d = orc.ray_dir(p, 0, 0)
+        first = int(r["hit_ids"][0, 0, 0])
+        if mode == 0:
+            assert first == 1, "mode 0: the enclosing proxy has no front face ahead; Gaussian 1 is the first hit"
+        else:
+            assert first == 0, "mode 1: the enclosing proxy's exit face is the closest hit"
+            ref = _radiance_real_sh(s.packed[0], 3, d)
+            assert np.abs(r["rgba"][0, :3] - ref).max() < 2e-5, (r["rgba"][0], ref)
+            away = np.asarray(s.packed[0][:3], np.float64) - np.array(u.to_world[12:15], np.float64)
+            other = _radiance_real_sh(s.packed[0], 3, away / np.linalg.norm(away))
+            assert np.abs(ref - other).max() > 1e-3  # the two directions give distinguishable colours
+
+
+def check_debug1_closest():
+    """debugMode 1 = the closest proxy face in (t_min, t_max), front or back,
+    coloured by radiance along the ray direction; black when there is none."""
+    raw, cams = _debug_scenes()
+    alpha, W, H = 0.05, 12, 10
+    checked = back = empty = 0
+    for pos, tgt in cams:
+        tw = gt.camera_look_at(pos, tgt)
+        u = gt.make_uniforms(tw, W, H, hfov_deg=50, alpha_min=alpha, k=6, debug_mode=1)
+        p = orc.params_from_uniforms(u, W, H)
+        ref = _fp64_hits(raw, alpha, u, W, H)
+        s = orc.OracleScene(raw, alpha)
+        r = orc.render(s, p, max_rounds=1, use_bvh=False)
+        for i, (hits, amb) in enumerate(ref):
+            if amb:
+                continue
+            got = int(r["hit_ids"][i, 0, 0])
+            if not hits:
+                assert got == orc.SENTINEL and np.all(r["rgba"][i, :3] == 0), i
+                empty += 1
+                continue
+            assert got == hits[0][1], (i, got, hits[:2])
+            back += not hits[0][2]
+            col = _radiance_real_sh(s.packed[got], 3, orc.ray_dir(p, i % W, i // W))
+            assert np.abs(r["rgba"][i, :3] - col).max() < 2e-5, (i, r["rgba"][i], col)
+            checked += 1
+    assert checked > 100 and back > 8 and empty >= 3, (checked, back, empty)
+
+
+BINS = [(0, (0, 0, 0)), (20, (0.5, 0.5, 0)), (40, (0.5, 0, 0.5)), (60, (0, 0.5, 0.5)), (80, (0.5, 0, 0)),
+        (100, (0, 0.5, 0)), (150, (0, 0, 0.5))]
+
+
+def _bin_colour(c):
+    """The hit-count colour table of GS_shader.ts:212-227: 0 is black, then
+    bins [1,20) [20,40) [40,60) [60,80) [80,100) [100,150), then grey."""
+    if c == 0:
+        return (0, 0, 0)
+    for hi, col in BINS[1:]:
+        if c < hi:
+            return col
+    return (0.5, 0.5, 0.5)
+
+
+def check_debug2_count():
+    """debugMode 2's count = the front-face crossings in (t_min, t_max)
+    (brute force in FP64); the colour is the table's bin of that count."""
+    raw, cams = _debug_scenes()
+    alpha, W, H = 0.05, 12, 10
+    checked = multi = 0
+    for pos, tgt in cams:
+        tw = gt.camera_look_at(pos, tgt)
+        u = gt.make_uniforms(tw, W, H, hfov_deg=50, alpha_min=alpha, k=6, debug_mode=2)
+        p = orc.params_from_uniforms(u, W, H)
+        ref = _fp64_hits(raw, alpha, u, W, H)
+        r = orc.render(orc.OracleScene(raw, alpha), p, max_rounds=1, use_bvh=False)
+        for i, (hits, amb) in enumerate(ref):
+            if amb:
+                continue
+            c = sum(1 for h in hits if h[2])
+            assert int(r["hit_ids"][i, 0, 0]) == c, (i, int(r["hit_ids"][i, 0, 0]), c)
+            assert tuple(float(x) for x in r["rgba"][i, :3]) == _bin_colour(c)
+            checked += 1
+            multi += c >= 3
+    assert checked > 100 and multi > 20, (checked, multi)
+
+
+def _count_scene(c, seed):
+    """c Gaussians on the camera axis whose proxies the 1x1 camera ray enters
+    once each (orientations redrawn until FP64 brute force finds exactly c
+    unambiguous front-face crossings)."""
+    rng = np.random.default_rng(seed)
+    tw = gt.camera_look_at([0.0, 0.0, 10.0], [0.0, 0.0, 0.0])
+    u = gt.make_uniforms(tw, 1, 1, hfov_deg=30, alpha_min=0.01, k=6, debug_mode=2)
+    zs = np.linspace(5.0, -8.0, max(c, 1))
+    for _ in range(20):
+        gs = [([0.0, 0.0, float(zs[j])], [-1.5] * 3, rng.normal(size=4), 0.4) for j in range(c)]
+        raw = np.concatenate([packed_record(*g) for g in gs]) if c else \
+            packed_record([0, 0, -50.0], [-1.5] * 3, [1, 0, 0, 0], -20.0)  # no proxy (opacity < alpha_min)
+        hits, amb = _fp64_hits(raw, 0.01, u, 1, 1)[0]
+        if not amb and sum(1 for h in hits if h[2]) == c:
+            return raw, u
+    raise AssertionError(f"no unambiguous scene for c={c}")
+
+
+def check_debug2_bins():
+    """Ray hit counts on each side of every bin boundary of the colour table."""
+    for c in (0, 1, 19, 20, 39, 40, 59, 60, 79, 80, 99, 100, 149, 150, 151):
+        raw, u = _count_scene(c, 100 + c)
+        r = orc.render(orc.OracleScene(raw, 0.01), orc.params_from_uniforms(u, 1, 1), max_rounds=1, use_bvh=False)
+        assert int(r["hit_ids"][0, 0, 0]) == c
+        assert tuple(float(x) for x in r["rgba"][0, :3]) == _bin_colour(c), (c, r["rgba"][0])
+
+
+def _anyhit_calls(ts, k, t_max):
+    """Any-hit invocations of one debugMode-2 trace for hits arriving in the
+    order ts (GS_shader.ts:201-210, 323-366; trace.glsl:183-215): a candidate
+    reaches the any-hit shader only below the current tmax; the shader
+    inserts it into k slots reset to t_max and commits (tmax = t) unless it
+    lies strictly below the k-th slot after insertion."""
+    kc = [t_max] * k
+    tmax, calls = t_max, 0
+    for t in ts:
+        if not t < tmax:
+            continue
+        calls += 1
+        h = t
+        for i in range(k):
+            if h < kc[i]:
+                kc[i], h = h, kc[i]
+        if not t < kc[k - 1]:
+            tmax = t
+    return calls
+
+
+def check_debug2_reference_range():
+    """The reference's debugMode-2 count depends on the arrival order of hits
+    (its BVH traversal order): over all orders it spans [min(k, n), n] for n
+    front-face crossings, reaching n only when they arrive far to near.  The
+    oracle (and kernel) count n, the upper end, for every order."""
+    raw, cams = _debug_scenes()
+    alpha, W, H, k = 0.05, 12, 10, 6
+    rng = np.random.default_rng(3)
+    pos, tgt = cams[0]
+    tw = gt.camera_look_at(pos, tgt)
+    u = gt.make_uniforms(tw, W, H, hfov_deg=50, alpha_min=alpha, k=k, debug_mode=2)
+    ref = _fp64_hits(raw, alpha, u, W, H)
+    r = orc.render(orc.OracleScene(raw, alpha), orc.params_from_uniforms(u, W, H), max_rounds=1, use_bvh=False)
+    seen = 0
+    for i, (hits, amb) in enumerate(ref):
+        ts = [h[0] for h in hits if h[2]]
+        if amb or len(ts) <= k:
+            continue
+        calls = [_anyhit_calls(list(rng.permutation(ts)), k, u.t_max) for _ in range(60)]
+        assert min(calls) >= k and max(calls) <= len(ts)
+        assert _anyhit_calls(sorted(ts), k, u.t_max) == k
+        assert _anyhit_calls(sorted(ts, reverse=True), k, u.t_max) == len(ts)
+        assert int(r["hit_ids"][i, 0, 0]) == len(ts)
+        seen += 1
+    assert seen > 10
+
+
 PINS = {
     "exp64": check_exp64, "log64": check_log64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir, "aa_jitter": check_aa_jitter,
     "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
     "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
     "composite": check_composite_closed_form, "relaunch": check_relaunch_window,
     "invariants": check_invariants_and_reeval,
+    "debug1_back_face": check_debug1_back_face, "debug1_closest": check_debug1_closest,
+    "debug2_count": check_debug2_count, "debug2_bins": check_debug2_bins,
+    "debug2_reference_range": check_debug2_reference_range,
 }
 
 
@@ -472,6 +706,12 @@ MUTATIONS = [
     (5, "relaunch"),          # relaunch without the 1e-4 step (GS_shader.ts:248)
     (6, "composite"),         # transmittance dropped from L += T*alpha*c
     (7, "proxy_tangent"),     # proxy scale s = sqrt(2 ln(o/alpha)) dropped (main.ts:84)
+    (8, "debug1_back_face"),  # debugMode 1 culling back faces (GS_shader.ts:292-294 passes no cull flag)
+    (8, "debug1_closest"),
+    (9, "debug1_closest"),    # debugMode 1 coloured along normalize(mu - o) instead of the ray (GS_shader.ts:318-320)
+    (9, "debug1_back_face"),
+    (10, "debug2_bins"),      # hit-count bin threshold off by one (c <= 20, GS_shader.ts:214)
+    (11, "debug2_count"),     # debugMode 2 counting back faces too (its trace culls them, GS_shader.ts:206-207)
 ]
 
 
