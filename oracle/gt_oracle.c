@@ -91,12 +91,14 @@ static void affine_inverse_f32(const float* m, float* out) {
   out[9] = -it.x; out[10] = -it.y; out[11] = -it.z;
 }
 
-uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, double alpha_min,
-                           uint32_t* proxy_ids, float* o2w, float* w2o) {
+/* The proxies of main.ts:79-94 from activated values act8 and positions at
+ * pos + i * pos_stride (the raw record or the packed record). */
+static uint32_t build_proxies_at(const float* pos, size_t pos_stride, const double* act8, uint32_t n,
+                                 double alpha_min, uint32_t* proxy_ids, float* o2w, float* w2o) {
   uint32_t np = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const double* a = act8 + (size_t)i * 8;
-    const float* r = raw + (size_t)i * GTO_RAW_FLOATS;
+    const float* r = pos + (size_t)i * pos_stride;
     double op = a[0];
     if (!(op > alpha_min)) continue;
     double s = sqrt(2 * gt_log64(op / alpha_min));
@@ -113,6 +115,30 @@ uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, dou
     proxy_ids[np++] = i;
   }
   return np;
+}
+
+uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, double alpha_min,
+                           uint32_t* proxy_ids, float* o2w, float* w2o) {
+  return build_proxies_at(raw, GTO_RAW_FLOATS, act8, n, alpha_min, proxy_ids, o2w, w2o);
+}
+
+/* A packed 320-B buffer (GS_ply.ts:198-205) holds the activated values only
+ * as f32: gaussianData's doubles (GS_ply.ts:225-235, read by main.ts:73-78)
+ * are not part of it, so the proxy builder's inputs are those f32 values
+ * widened to double. */
+void gto_activate_packed(const float* packed, uint32_t n, double* act8) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* o = packed + (size_t)i * GTO_PACKED_FLOATS;
+    double* a = act8 + (size_t)i * 8;
+    a[0] = o[12];
+    for (int j = 0; j < 3; ++j) a[1 + j] = o[4 + j];
+    for (int j = 0; j < 4; ++j) a[4 + j] = o[8 + j];
+  }
+}
+
+uint32_t gto_build_proxies_packed(const float* packed, const double* act8, uint32_t n, double alpha_min,
+                                  uint32_t* proxy_ids, float* o2w, float* w2o) {
+  return build_proxies_at(packed, GTO_PACKED_FLOATS, act8, n, alpha_min, proxy_ids, o2w, w2o);
 }
 
 /* ---------------- shading pieces ---------------- */

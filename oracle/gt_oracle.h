@@ -8,7 +8,8 @@
  * Restated reference files:
  *   activations          src/GS_ply.ts:225-235
  *   320-B record         src/GS_ply.ts:198-205, src/GS_packing.ts:146-231
- *   proxy filter/xform   src/main.ts:73-94 (wgpu-matrix mat3.fromQuat / mat4.mul)
+ *   proxy filter/xform   src/main.ts:73-94 (wgpu-matrix mat3.fromQuat / mat4.mul),
+ *                        from the JS doubles or from a packed buffer's f32 values
  *   world-to-object      webrtx/bvh/src/lib.rs:501-505 (glam Affine3A::inverse)
  *   ray generation       src/GS_shader.ts:275-301
  *   round loop/composite src/GS_shader.ts:192-273
@@ -61,6 +62,12 @@ void gto_activate(const float* raw, uint32_t n, float* packed, double* act8);
 /* main.ts:73-94 + lib.rs:501-505.  Returns the number of proxies. */
 uint32_t gto_build_proxies(const float* raw, const double* act8, uint32_t n, double alpha_min,
                            uint32_t* proxy_ids, float* o2w, float* w2o);
+
+/* The same from packed 320-B records (GS_ply.ts:198-205): act8 from the
+ * f32 fields (gto_activate_packed), positions from the record. */
+void gto_activate_packed(const float* packed, uint32_t n, double* act8);
+uint32_t gto_build_proxies_packed(const float* packed, const double* act8, uint32_t n, double alpha_min,
+                                  uint32_t* proxy_ids, float* o2w, float* w2o);
 
 /* Render.  ray_list (optional) selects rays by linear index y*W+x; outputs
  * With p->antialias and accumulated_frames > 0, rgba holds the running

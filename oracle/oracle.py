@@ -39,6 +39,10 @@ _P = C.c_void_p
 _lib.gto_activate.argtypes = [_P, C.c_uint32, _P, _P]
 _lib.gto_build_proxies.argtypes = [_P, _P, C.c_uint32, C.c_double, _P, _P, _P]
 _lib.gto_build_proxies.restype = C.c_uint32
+_lib.gto_activate_packed.argtypes = [_P, C.c_uint32, _P]
+_lib.gto_activate_packed.restype = None
+_lib.gto_build_proxies_packed.argtypes = [_P, _P, C.c_uint32, C.c_double, _P, _P, _P]
+_lib.gto_build_proxies_packed.restype = C.c_uint32
 _lib.gto_render.argtypes = [C.POINTER(Scene), C.POINTER(Params), C.c_int, C.c_int, _P, C.c_uint32,
                             _P, _P, _P, C.c_uint32, _P, _P, _P]
 _lib.gto_reeval_f64.argtypes = [C.POINTER(Scene), C.POINTER(Params), _P, C.c_uint32, _P, C.c_uint32, _P, _P,
@@ -91,13 +95,33 @@ def build_proxies(raw, act, alpha_min):
     return ids[:npx].copy(), o2w[:npx].copy(), w2o[:npx].copy()
 
 
-class OracleScene:
-    """Activated records + proxies of a PLY-raw scene, built by the oracle."""
+def build_proxies_packed(packed, alpha_min):
+    """Proxies of packed 320-B records (n x 80 floats) from their f32 fields."""
+    packed = np.ascontiguousarray(packed, np.float32).reshape(-1, PACKED)
+    n = packed.shape[0]
+    act = np.zeros((n, 8), np.float64)
+    _lib.gto_activate_packed(_p(packed), n, _p(act))
+    ids = np.zeros(n, np.uint32)
+    o2w = np.zeros((n, 12), np.float32)
+    w2o = np.zeros((n, 12), np.float32)
+    npx = _lib.gto_build_proxies_packed(_p(packed), _p(act), n, float(alpha_min), _p(ids), _p(o2w), _p(w2o))
+    return act, ids[:npx].copy(), o2w[:npx].copy(), w2o[:npx].copy()
 
-    def __init__(self, raw, alpha_min):
-        self.raw = np.ascontiguousarray(raw, np.float32)
-        self.packed, self.act = activate(self.raw)
-        self.proxy_ids, self.o2w, self.w2o = build_proxies(self.raw, self.act, alpha_min)
+
+class OracleScene:
+    """Activated records + proxies of a PLY-raw scene (or, with packed=, of
+    packed 320-B records whose proxies come from the f32 fields), built by
+    the oracle."""
+
+    def __init__(self, raw=None, alpha_min=0.2, packed=None):
+        if packed is not None:
+            self.raw = None
+            self.packed = np.ascontiguousarray(packed, np.float32).reshape(-1, PACKED)
+            self.act, self.proxy_ids, self.o2w, self.w2o = build_proxies_packed(self.packed, alpha_min)
+        else:
+            self.raw = np.ascontiguousarray(raw, np.float32)
+            self.packed, self.act = activate(self.raw)
+            self.proxy_ids, self.o2w, self.w2o = build_proxies(self.raw, self.act, alpha_min)
         self._s = Scene(self.packed.shape[0], _p(self.packed), len(self.proxy_ids), _p(self.proxy_ids),
                         _p(self.o2w), _p(self.w2o), None)
 

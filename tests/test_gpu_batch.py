@@ -6,6 +6,8 @@ gt_tuning setting (tile kernel or persistent kernel, ray-tile shape, LDS
 stack depth, scheduling thresholds) gives bit-identical outputs; hipGraph
 replays equal eager launches.
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -149,3 +151,80 @@ def test_batch_rejects_mixed_k():
     frames = [dict(rgba=t.data_ptr()) for t in imgs]
     with pytest.raises(gt.GTError, match="shares k"):
         gt.Batch(s, us, SIZES, frames)
+
+
+def test_graph_replay_after_scene_rebuild():
+    """A captured batch graph holds the scene's device arrays by value.  After
+    a rebuild moves them (gt_scene_set_alpha_min; a refit whose proxy set
+    changed, which rebuilds on the GPU) the next replay must render the new
+    scene: equal, camera by camera, to gt_render."""
+    print("case batch graph after rebuilds", flush=True)
+    s, c = _scene()
+    us = _cams(c)
+    MR = 64
+    _same(_render_batch(s, us, MR, use_graph=True, replays=1)[0], _single(s, us, MR))
+    dev = torch.device("cuda:0")
+    frames, bufs = [], []
+    for u, (w, h) in zip(us, SIZES):
+        b = dict(rgba=torch.zeros((h, w, 4), device=dev),
+                 hits=torch.zeros((w * h, MR, u.k), dtype=torch.int32, device=dev),
+                 acc=torch.zeros((w * h, MR), dtype=torch.int32, device=dev),
+                 rounds=torch.zeros(w * h, dtype=torch.int32, device=dev))
+        bufs.append(b)
+        frames.append(dict(rgba=b["rgba"].data_ptr(), hit_ids=b["hits"].data_ptr(), accept=b["acc"].data_ptr(),
+                           n_rounds=b["rounds"].data_ptr(), max_rounds=MR))
+    batch = gt.Batch(s, us, SIZES, frames)
+    st = torch.cuda.current_stream().cuda_stream
+
+    def replay():
+        for b in bufs:
+            b["rgba"].fill_(-1.0)
+        batch.render(st, use_graph=True)
+        torch.cuda.synchronize()
+        return [{k: v.cpu().numpy().copy() for k, v in b.items()} for b in bufs]
+
+    _same(replay(), _single(s, us, MR))
+    s.set_alpha_min(0.05)  # re-filter: every device array is rebuilt
+    _same(replay(), _single(s, us, MR))
+    raw = gt.generate_raw(n=60000, seed=21, config="c2", n_clusters=6)
+    raw2 = raw.copy()
+    raw2[::7, 51] = -9.0  # opacity far below alpha_min: the proxy set changes
+    s2 = gt.Scene(raw, alpha_min=0.05)
+    b2 = gt.Batch(s2, us, SIZES, frames)
+    b2.render(st, use_graph=True)
+    torch.cuda.synchronize()
+    x = torch.from_numpy(raw2).to(dev)
+    assert s2.refit_device(x.data_ptr(), st) is True  # fell back to a GPU rebuild
+    for b in bufs:
+        b["rgba"].fill_(-1.0)
+    b2.render(st, use_graph=True)
+    torch.cuda.synchronize()
+    _same([{k: v.cpu().numpy().copy() for k, v in b.items()} for b in bufs], _single(s2, us, MR))
+    s3 = gt.Scene(raw2, alpha_min=0.05)
+    _same([{k: v.cpu().numpy().copy() for k, v in b.items()} for b in bufs], _single(s3, us, MR))
+    b2.close()
+    batch.close()
+    for sc in (s, s2, s3):
+        sc.close()
+
+
+def test_first_render_of_a_process_is_a_graph_batch():
+    """The face table is uploaded at scene creation, outside any capture: a
+    process whose first render is a captured batch renders correctly."""
+    import subprocess
+    import sys
+    print("case graph batch as the first render of a process", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = (
+        "import sys; sys.path[:0] = %r\n"
+        "import numpy as np, torch, gtrace as gt\n"
+        "from test_gpu_batch import _scene, _cams, _render_batch, _single, _same\n"
+        "s, c = _scene(); us = _cams(c)\n"
+        "out = _render_batch(s, us, 64, use_graph=True, replays=2)\n"
+        "for o in out: _same(o, _single(s, us, 64))\n"
+        "print('first-render graph batch ok', flush=True)\n"
+    ) % ([here, os.path.join(os.path.dirname(here), "webgpu-gaussiantracer_amd"),
+          os.path.join(os.path.dirname(here), "oracle")],)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "first-render graph batch ok" in r.stdout

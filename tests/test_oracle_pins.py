@@ -10,6 +10,7 @@ test_mutation_detected plants plausible mistakes in the oracle (through its
 mutation switch) and requires the matching pin to fail.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -182,14 +183,20 @@ def check_aa_jitter():
     assert _reference_pixel_offset(5, 3, 33, 1) != _reference_pixel_offset(5, 3, 33, 2)
 
 
-def check_proxy_tangent():
+def check_proxy_tangent(packed_input=False):
     """Every proxy face is tangent to the Mahalanobis sphere of radius
     s = sqrt(2 ln(opacity / alpha_min)) around the mean (main.ts:84-87),
-    with Sigma = R S^2 R^T, R the standard rotation of the quaternion."""
+    with Sigma = R S^2 R^T, R the standard rotation of the quaternion.
+    packed_input: the proxies of a packed 320-B buffer, built from its f32
+    opacity, scale and quaternion (the JS doubles are not in the buffer)."""
     raw = gt.generate_raw(n=200, seed=11, config="c1")
     alpha = 0.05
     packed, act = orc.activate(raw)
-    ids, o2w, w2o = orc.build_proxies(raw, act, alpha)
+    if packed_input:
+        act, ids, o2w, w2o = orc.build_proxies_packed(packed, alpha)
+        assert np.array_equal(act[:, 1:4], packed[:, 4:7].astype(np.float64))
+    else:
+        ids, o2w, w2o = orc.build_proxies(raw, act, alpha)
     assert len(ids) > 100
     V, I = ico_geometry()
     for j, gid in enumerate(ids[:60]):
@@ -681,7 +688,8 @@ def check_debug2_reference_range():
 
 PINS = {
     "exp64": check_exp64, "log64": check_log64, "icosahedron": check_icosahedron, "ray_dir": check_ray_dir, "aa_jitter": check_aa_jitter,
-    "proxy_tangent": check_proxy_tangent, "tri_vs_mt": check_tri_vs_mt,
+    "proxy_tangent": check_proxy_tangent, "proxy_tangent_packed": lambda: check_proxy_tangent(packed_input=True),
+    "tri_vs_mt": check_tri_vs_mt,
     "kbuffer_bruteforce": check_kbuffer_bruteforce, "response": check_response, "sh": check_sh,
     "composite": check_composite_closed_form, "relaunch": check_relaunch_window,
     "invariants": check_invariants_and_reeval,
@@ -706,6 +714,7 @@ MUTATIONS = [
     (5, "relaunch"),          # relaunch without the 1e-4 step (GS_shader.ts:248)
     (6, "composite"),         # transmittance dropped from L += T*alpha*c
     (7, "proxy_tangent"),     # proxy scale s = sqrt(2 ln(o/alpha)) dropped (main.ts:84)
+    (7, "proxy_tangent_packed"),
     (8, "debug1_back_face"),  # debugMode 1 culling back faces (GS_shader.ts:292-294 passes no cull flag)
     (8, "debug1_closest"),
     (9, "debug1_closest"),    # debugMode 1 coloured along normalize(mu - o) instead of the ray (GS_shader.ts:318-320)
@@ -723,3 +732,35 @@ def test_mutation_detected(mutation, pin):
             PINS[pin]()
     finally:
         orc.set_mutation(0)
+
+
+def test_packed_buffer_proxies_vs_js_doubles():
+    """How the proxies of a packed buffer (gt_scene_create_packed) differ from
+    those of the JS doubles (gt_scene_create_raw, main.ts:73-94): the same
+    Gaussians get proxies; each transform column agrees to 1e-4 of its norm
+    (the inputs are rounded to f32 once more, and the proxy scale
+    sqrt(2 ln(opacity / alpha_min)) amplifies the opacity's rounding for
+    opacities near alpha_min: 1.3e-5 at worst on C1, 4e-7 on pc_short.ply);
+    hit lists agree on every ray of
+    pc_short.ply and on all but a handful of the 65,536 C1 rays, where a ray
+    grazes a proxy face."""
+    for name, raw, alpha, W, H, cam, k, Tmin, max_diff_rays in [
+            ("pc_short", gt.read_ply(os.path.join(os.path.dirname(__file__), "golden", "pc_short.ply"))[0],
+             0.01, 64, 64, None, 4, 1e-3, 0),
+            ("c1", gt.generate_raw(seed=0, config="c1"), 0.1, 256, 256, ([0, 0, 8.0], [0, 0, 0.0]), 6, 0.03, 16)]:
+        a = orc.OracleScene(raw, alpha)
+        b = orc.OracleScene(packed=a.packed, alpha_min=alpha)
+        assert np.array_equal(a.proxy_ids, b.proxy_ids), name
+        for m1, m2 in ((a.o2w, b.o2w), (a.w2o, b.w2o)):
+            c1 = m1.astype(np.float64).reshape(-1, 4, 3)[:, :3]
+            c2 = m2.astype(np.float64).reshape(-1, 4, 3)[:, :3]
+            rel = np.abs(c1 - c2).max(axis=2) / np.linalg.norm(c1, axis=2)
+            assert rel.max() < 1e-4, (name, rel.max())
+        if cam is None:
+            c = raw[:, :3].mean(axis=0)
+            cam = (list(c + np.array([0.0, 0.0, 3.0])), list(c))
+        u = gt.make_uniforms(gt.camera_look_at(*cam), W, H, hfov_deg=60, alpha_min=alpha, k=k, T_min=Tmin)
+        p = orc.params_from_uniforms(u, W, H)
+        ra, rb = orc.render(a, p, max_rounds=24), orc.render(b, p, max_rounds=24)
+        differ = int((ra["hit_ids"] != rb["hit_ids"]).any(axis=(1, 2)).sum())
+        assert differ <= max_diff_rays, (name, differ)

@@ -422,15 +422,20 @@ class Scene:
         return out
 
     def render_device(self, u, width, height, rgba_ptr, stream_ptr=None, stats_ptr=None, comp_ids_ptr=None,
-                      n_comp_ptr=None, comp_cap=0, antialiasing=False):
+                      n_comp_ptr=None, comp_cap=0, antialiasing=False, hit_ids_ptr=None, accept_ptr=None,
+                      n_rounds_ptr=None, T_ptr=None, max_rounds=0):
         """Asynchronous traceRays into caller-owned device memory (GT_BUF_DEVICE),
         optionally writing the backward checkpoint (or, with antialiasing,
-        averaging the jittered frame into rgba)."""
+        averaging the jittered frame into rgba), and the per-ray outputs of
+        gt_frame: hit lists (R x max_rounds x k ids), accept masks
+        (R x max_rounds), round counts and final T (R each)."""
         f = Frame()
         f.rgba = rgba_ptr
         f.antialiasing = 1 if antialiasing else 0
         f.stats = stats_ptr
         f.comp_ids, f.n_comp, f.comp_cap = comp_ids_ptr, n_comp_ptr, comp_cap
+        f.hit_ids, f.accept, f.n_rounds, f.T = hit_ids_ptr, accept_ptr, n_rounds_ptr, T_ptr
+        f.max_rounds = max_rounds
         _check(_lib.gt_render(self._h, C.byref(u), width, height, C.byref(f), 1, stream_ptr))
 
 
