@@ -307,6 +307,22 @@ int gt_batch_render(gt_batch* b, int use_graph, void* hip_stream);
 int gt_batch_rays(const gt_batch* b, uint64_t* rays);
 void gt_batch_destroy(gt_batch* b);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gt_batch_create + gt_batch_render (no graph) as one
+ * call that allocates nothing.  Same arguments, checks and results: each
+ * camera's outputs equal gt_render on that camera.  The batch's scratch
+ * (the camera table the render kernels read and the persistent renderer's
+ * work counter) lives in a file-scope device array sized for
+ * GT_NOALLOC_MAX_CAMERAS cameras (about 54 KB), which every GPU the library
+ * runs on holds until the process ends; more cameras is GT_E_LIMIT.  All
+ * calls on one GPU share that array, so calls that overlap on one GPU
+ * (other streams, other threads) race: order them on one stream or
+ * synchronise between them.  Asynchronous on hip_stream. */
+#define GT_NOALLOC_MAX_CAMERAS 256
+int gt_batch_render_noalloc(gt_scene* s, uint32_t n, const gt_uniforms* u, const uint32_t* widths,
+                            const uint32_t* heights, const gt_frame* outs, void* hip_stream);
+
 /* ---- autotuner (no counterpart in the reference) ----
  * Times the result-neutral gt_tuning candidates on the caller's workload
  * (iters timed renders each, after one warm-up) and stores the fastest in

@@ -30,7 +30,7 @@ CXX_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
-HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip", "gt_image.hip"]
+HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip", "gt_image.hip", "gt_noalloc.hip"]
 CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp", "gt_png.cpp"]
 
 

@@ -121,6 +121,7 @@ _lib.gt_scene_get_tuning.argtypes = [_P, C.POINTER(Tuning)]
 _lib.gt_batch_create.argtypes = [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(_P)]
 _lib.gt_batch_render.argtypes = [_P, C.c_int, _P]
 _lib.gt_batch_rays.argtypes = [_P, C.POINTER(C.c_uint64)]
+_lib.gt_batch_render_noalloc.argtypes = [_P, C.c_uint32, _P, _P, _P, _P, _P]
 _lib.gt_batch_destroy.argtypes = [_P]
 _lib.gt_batch_destroy.restype = None
 _lib.gt_autotune_frame.argtypes = [_P, C.POINTER(Uniforms), C.c_uint32, C.c_uint32, _P, C.c_uint32, _P,
@@ -143,7 +144,7 @@ TONEMAP_IDENTITY, TONEMAP_EXPOSURE, TONEMAP_EXPOSURE_GAMMA = 0, 1, 2
 EXPORTED = [
     "gt_uniforms_init", "gt_camera_look_at", "gt_scene_create_raw", "gt_scene_create_packed",
     "gt_scene_set_alpha_min", "gt_scene_create_device", "gt_scene_create_raw_gpu", "gt_scene_update_device", "gt_scene_refit_device", "gt_scene_destroy", "gt_scene_get_info", "gt_scene_get_host_data",
-    "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy",
+    "gt_render", "gt_backward", "gt_autotune_frame", "gt_autotune_batch", "gt_adam_step", "gt_batch_create", "gt_batch_render", "gt_batch_rays", "gt_batch_destroy", "gt_batch_render_noalloc",
     "gt_scene_set_tuning", "gt_scene_get_tuning", "gt_host_prepare", "gt_debug_face_test", "gt_generate_scene", "gt_ply_read", "gt_free", "gt_tonemap", "gt_write_png", "gt_last_error", "gt_version",
 ]
 
@@ -481,6 +482,28 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+NOALLOC_MAX_CAMERAS = 256
+
+
+def batch_render_noalloc(scene, uniforms, sizes, frames, stats_ptr=None, stream_ptr=None):
+    """The cameras of a Batch rendered by one call that allocates no device
+    memory (gt_batch_render_noalloc); frames as for Batch.  Asynchronous."""
+    n = len(uniforms)
+    U = (Uniforms * n)(*uniforms)
+    Wa = (C.c_uint32 * n)(*[w for w, _ in sizes])
+    Ha = (C.c_uint32 * n)(*[h for _, h in sizes])
+    F = (Frame * n)()
+    for i, fr in enumerate(frames):
+        F[i].rgba = fr["rgba"]
+        F[i].hit_ids = fr.get("hit_ids")
+        F[i].accept = fr.get("accept")
+        F[i].n_rounds = fr.get("n_rounds")
+        F[i].T = fr.get("T")
+        F[i].max_rounds = fr.get("max_rounds", 0)
+    F[0].stats = stats_ptr
+    _check(_lib.gt_batch_render_noalloc(scene.handle, n, U, Wa, Ha, F, stream_ptr))
 
 
 def _autotune_dict(r):
