@@ -14,12 +14,19 @@ roofline: algorithmic HBM bytes per frame
 (R rays, H_eval evaluated k-buffer entries, H_comp composited entries: the
 RGBA write, the 44-B mu/scale/rotation/opacity read per evaluated hit and
 the 192-B SH read per composited hit), divided by the kernel's average
-duration measured with HIP events on the launch stream.
+duration measured with HIP events on the launch stream.  roofline.traffic is
+the kernel's FETCH_SIZE + WRITE_SIZE per launch, measured in the same run by
+rocprofv3 --pmc child processes and scaled by what the counters read for a
+plain 1 GiB device copy (null, with the reason, when rocprofv3 is missing or
+fails).
 
 fwd_bwd: forward with the backward checkpoint + dL/dRGBA of a squared-error
-loss + the hand-written backward, on the same frame (and the recompute mode).
+loss + the hand-written backward, on the same frame (and the recompute mode),
+with a roofline per kernel group from the bytes those kernels must move.
 batched: 64 cameras of uneven resolution in one launch replayed from a
-hipGraph, against the same cameras rendered one launch at a time.
+hipGraph, for alpha_min 0.01 and 0.1 and k 4, 8, 16, 32, each with the ray
+tile shape the autotuner picks and checked bit for bit against the same
+cameras rendered one launch at a time (the plain kernel at that k).
 fwd_bwd_c4: the same forward + backward on 200k single-cluster Gaussians, 800x800.
 c5: 3M Gaussians, 16 views at 1280x720, forward with checkpoint + backward
 per view with gradients accumulated across views, peak device memory (and
@@ -31,8 +38,10 @@ update) on the 200k-Gaussian 800x800 workload: refit of the host-built tree
 (the default), and a GPU rebuild for comparison.
 
 cpu_baseline: the oracle (the project's C restatement of the reference path,
-BVH variant, all host threads up to 16) on a seeded 1/3 sample of the rays.
+BVH variant) on every core the process may use, on a seeded 1/3 sample of
+the rays.
 """
+import argparse
 import argparse
 import json
 import os
@@ -43,13 +52,168 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "webgpu-gaussiantracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_forward launch of the default frame (1M Gaussians,
-# 1920x1080, k 16), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# this script scaled by the counters' reading for a 1 GiB device copy
-# (x2.000 and x1.000): profiles/k_forward_1M_1080p_k16.md.  Reported as
-# roofline.traffic for that workload only.
-TRAFFIC_DEFAULT_FRAME = {"fetch_bytes": 543963750, "write_bytes": 171515637}
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
+FWD_KERNEL = "k_forward<16, 0, false, false, false, true, true> (packet traversal, persistent kernel)"
+
+
+def _counter_csv_rows(outdir):
+    """Rows of every rocprofv3 counter_collection.csv under outdir."""
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def _per_dispatch(rows, counter):
+    """[(kernel name, value)] per dispatch in dispatch order, summed over the
+    counter's rows of that dispatch."""
+    acc = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        key = int(r.get("Dispatch_Id") or r.get("Correlation_Id") or 0)
+        name, v = acc.get(key, (r.get("Kernel_Name", ""), 0.0))
+        acc[key] = (name, v + float(r.get("Counter_Value") or 0.0))
+    return [acc[k] for k in sorted(acc)]
+
+
+def measure_traffic(args, timeout_s=120):
+    """HBM bytes of the forward kernel (first two plain launches of the
+    default frame) and of the backward's kernels (k_colors, k_backward,
+    k_finalize of one checkpoint-mode call), from rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE child runs of this script (--traffic-probe), each counter
+    scaled by its reading for a plain 1 GiB device copy (--copy-calib).  Run
+    before this process touches the GPU; returns (dict, None) or (None,
+    reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                          else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    base = [sys.executable, os.path.abspath(__file__)]
+    probe = base + ["--traffic-probe", "--n", str(args.n), "--width", str(args.width), "--height",
+                    str(args.height), "--k", str(args.k)]
+    calib = base + ["--copy-calib"]
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="gt_traffic_")
+    env = dict(os.environ, TMPDIR=tempfile.gettempdir())
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            for name, cmd in (("probe", probe), ("calib", calib)):
+                d = os.path.join(tmp, f"{name}_{counter}")
+                r = subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "-f", "csv", "--"] + cmd,
+                                   capture_output=True, text=True, timeout=timeout_s, env=env)
+                if r.returncode != 0:
+                    return None, f"rocprofv3 --pmc {counter} ({name}) exited {r.returncode}: {r.stderr[-300:]}"
+                out[(name, counter)] = _per_dispatch(_counter_csv_rows(d), counter)
+        d = os.path.join(tmp, "probe_trace")
+        r = subprocess.run([prof, "--kernel-trace", "-d", d, "-o", "run", "-f", "csv", "--"] + probe,
+                           capture_output=True, text=True, timeout=timeout_s, env=env)
+        if r.returncode != 0:
+            return None, f"rocprofv3 --kernel-trace (probe) exited {r.returncode}: {r.stderr[-300:]}"
+        out["trace"] = _kernel_trace_ms(d)
+    except subprocess.TimeoutExpired:
+        return None, f"rocprofv3 pass took more than {timeout_s} s"
+    except OSError as e:
+        return None, f"rocprofv3 could not run: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    res = {"copy_bytes": 1 << 30}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = [v for n, v in out[("calib", counter)] if "copy" in n.lower()] or \
+            [max((v for _, v in out[("calib", counter)]), default=0.0)]
+        kib = sum(cal) / len(cal)
+        if kib <= 0:
+            return None, f"no {counter} reading for the calibration copy"
+        scale = (1 << 30) / (kib * 1024.0)  # the copy reads and writes 1 GiB
+        fwd = [v for n, v in out[("probe", counter)] if "k_forward<" in n][:2]
+        if len(fwd) < 2:
+            return None, f"no {counter} reading for k_forward"
+        res[counter] = {"scale": round(scale, 4), "calib_kib": round(kib, 1),
+                        "k_forward_bytes": int(sum(fwd) / len(fwd) * 1024.0 * scale)}
+        bwd = {}
+        for kname in ("k_colors", "k_backward", "k_finalize"):
+            v = [x for n, x in out[("probe", counter)] if kname in n]
+            bwd[kname] = int(sum(v) * 1024.0 * scale)
+        res[counter]["backward_bytes"] = bwd
+    res["kernel_ms"] = out["trace"]
+    return res, None
+
+
+def _kernel_trace_ms(outdir):
+    """{kernel: [duration ms per dispatch]} from rocprofv3 kernel_trace.csv files."""
+    import csv
+    import glob
+    ms = {}
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                name = r.get("Kernel_Name", "")
+                for key in ("k_forward<", "k_colors", "k_backward", "k_finalize", "k_zero_f32"):
+                    if key in name:
+                        d = (float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) * 1e-6
+                        ms.setdefault(key.rstrip("<"), []).append(round(d, 4))
+    return ms
+
+
+def traffic_probe(args):
+    """--traffic-probe: two plain frames of the default workload, then one
+    forward with the backward checkpoint and one backward (what
+    measure_traffic profiles)."""
+    import torch
+    import gtrace as gt
+    dev = torch.device("cuda:0")
+    W, H, K = args.width, args.height, args.k
+    raw = gt.generate_raw(n=args.n, seed=0, config="c2")
+    scene = gt.Scene(raw, alpha_min=0.01)
+    pos, tgt = camera_c2()
+    u = gt.make_uniforms(gt.camera_look_at(pos, tgt), W, H, hfov_deg=50.0, t_min=1e-3, t_max=1e5, T_min=1e-3,
+                         alpha_min=0.01, k=K, sh_degree_max=3)
+    sp = torch.cuda.current_stream().cuda_stream
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    for _ in range(2):
+        scene.render_device(u, W, H, rgba.data_ptr(), sp)
+    R = W * H
+    ncomp = torch.empty(R, dtype=torch.int32, device=dev)
+    scene.render_device(u, W, H, rgba.data_ptr(), sp, n_comp_ptr=ncomp.data_ptr())
+    torch.cuda.synchronize()
+    cap = 1 << max(0, int(ncomp.max().item()) - 1).bit_length()
+    cids = torch.empty((R, cap), dtype=torch.int32, device=dev)
+    scene.render_device(u, W, H, rgba.data_ptr(), sp, comp_ids_ptr=cids.data_ptr(), n_comp_ptr=ncomp.data_ptr(),
+                        comp_cap=cap)
+    g = torch.ones_like(rgba)
+    grads = torch.zeros((raw.shape[0], gt.GT_RAW_FLOATS), dtype=torch.float32, device=dev)
+    gt.backward(scene, u, W, H, g.data_ptr(), grads.data_ptr(), cids.data_ptr(), ncomp.data_ptr(), cap, None, 0, sp)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done"}), flush=True)
+
+
+def cpu_info():
+    """(threads the process may run on, nproc, CPU model, cgroup CPU quota)."""
+    import math
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(math.floor(quota))))
+    return threads, os.cpu_count(), model, quota, aff
 
 
 def camera_c2(seed=1):
@@ -73,48 +237,44 @@ def cameras_c3(n, seed=1):
     return [list(25.0 * d) for d in dirs], sizes
 
 
-def bench_batched(gt, scene, torch, dev, sp, K, n_cams, steps=3):
+def bench_batched(gt, raw, scene, torch, dev, sp, n_cams):
     """Batched forward: n_cams cameras of uneven resolution rendered by one
-    launch replayed from a hipGraph, against the same cameras rendered one
-    gt_render launch at a time (the plain single-camera kernel)."""
+    launch replayed from a hipGraph, for alpha_min 0.01 and 0.1 and
+    hit_array_size 4, 8, 16, 32; for each, the ray-tile shape (4, 8, 16 or
+    32 pixels wide) is picked by gtrace.autotune_batch_k, and the batch is
+    checked bit for bit against the same cameras rendered one gt_render
+    launch at a time at that k (the plain kernel), whose time is reported
+    beside it.  The headline entry is alpha_min 0.01, k 16."""
     poses, sizes = cameras_c3(n_cams)
-    us = [gt.make_uniforms(gt.camera_look_at(p, [0.0, 0.0, 0.0]), w, h, hfov_deg=50.0, T_min=1e-3, alpha_min=0.01,
-                           k=K) for p, (w, h) in zip(poses, sizes)]
-    bufs = [torch.empty((h, w, 4), dtype=torch.float32, device=dev) for w, h in sizes]
     rays = sum(w * h for w, h in sizes)
-    stream = torch.cuda.current_stream()
-
-    def timed(fn, n):
-        fn()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(n):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / n
-
-    saved = scene.get_tuning()
-    scene.set_tuning()  # library default = the plain tile kernel
-
-    def plain():
-        for u, (w, h), b in zip(us, sizes, bufs):
-            scene.render_device(u, w, h, b.data_ptr(), sp)
-    ms_plain = timed(plain, steps)
-    batch = gt.Batch(scene, us, sizes, [dict(rgba=b.data_ptr()) for b in bufs])
-    tuned = gt.autotune_batch(scene, batch, use_graph=True, iters=1, stream_ptr=sp)
-    ms_batch = timed(lambda: batch.render(sp, use_graph=True), steps)
-    batch.close()
-    scene.set_tuning(**{k: v for k, v in saved.items()})
+    sweep, fastest = [], {}
+    for alpha in (0.01, 0.1):
+        sc = scene if alpha == 0.01 else gt.Scene(raw, alpha_min=alpha)
+        us = [gt.make_uniforms(gt.camera_look_at(p, [0.0, 0.0, 0.0]), w, h, hfov_deg=50.0, T_min=1e-3,
+                               alpha_min=alpha, k=16) for p, (w, h) in zip(poses, sizes)]
+        res, fk = gt.autotune_batch_k(sc, us, sizes, ks=(4, 8, 16, 32), tile_ws=(4, 8, 16, 32), iters=2,
+                                      stream_ptr=sp)
+        fastest[str(alpha)] = fk
+        for r in res:
+            sweep.append({"alpha_min": alpha, "k": r["k"], "mrays": round(rays / r["ms"] / 1e3, 3),
+                          "ms_per_frame": round(r["ms"], 3), "tile_w": r["tile_w"],
+                          "plain_mrays": round(rays / r["plain_ms"] / 1e3, 3),
+                          "plain_ms_per_frame": round(r["plain_ms"], 3),
+                          "speedup_vs_plain": round(r["plain_ms"] / r["ms"], 3),
+                          "equal_to_plain": r["equal_to_plain"]})
+        if sc is not scene:
+            sc.close()
+    head = [r for r in sweep if r["alpha_min"] == 0.01 and r["k"] == 16][0]
     return {
-        "workload": f"{n_cams} cameras on a sphere of radius 25, resolutions uniform from "
-                    f"{BATCH_SIZES}, hFov 50, alpha_min 0.01, k {K}, T_min 0.001, one launch per frame "
-                    f"replayed from a hipGraph",
-        "rays": rays, "mrays": round(rays / ms_batch / 1e3, 3), "ms_per_frame": round(ms_batch, 3),
-        "plain_mrays": round(rays / ms_plain / 1e3, 3), "plain_ms_per_frame": round(ms_plain, 3),
-        "speedup_vs_plain": round(ms_plain / ms_batch, 3), "tuning": tuned["best"], "steps": steps,
+        "workload": f"{n_cams} cameras on a sphere of radius 25, resolutions uniform from {BATCH_SIZES}, hFov 50, "
+                    f"T_min 0.001; one launch per frame replayed from a hipGraph (persistent packet kernel, ray "
+                    f"tile shape chosen by the autotuner from widths 4, 8, 16, 32); headline alpha_min 0.01, k 16",
+        "rays": rays, "mrays": head["mrays"], "ms_per_frame": head["ms_per_frame"],
+        "plain_mrays": head["plain_mrays"], "plain_ms_per_frame": head["plain_ms_per_frame"],
+        "speedup_vs_plain": head["speedup_vs_plain"], "tuning": {"path": 4, "tile_w": head["tile_w"]},
+        "sweep": sweep, "fastest_k": fastest,
+        "note": "plain = the same cameras as one gt_render launch each at the same k; every batch equals its plain "
+                "renders bit for bit; k changes the image, so the fastest k is reported, not imposed",
     }
 
 
@@ -166,6 +326,34 @@ def _events_ms(torch, stream, fn, n):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / n
+
+
+def backward_counts(torch, cids, ncomp, W, H, n):
+    """From a frame's checkpoint: composited hits H_comp, rays with hits
+    R_hit, distinct composited Gaussians N_touched, and distinct (16x16
+    pixel block, Gaussian) pairs P (the backward flushes its per-block
+    accumulators once per pair)."""
+    R = W * H
+    cap = cids.shape[1]
+    nc = ncomp.clamp(max=cap)
+    valid = torch.arange(cap, device=cids.device)[None, :] < nc[:, None]
+    pix = torch.arange(R, device=cids.device, dtype=torch.int64)[:, None].expand(R, cap)[valid]
+    gid = cids[valid].to(torch.int64)
+    bx = (W + 15) // 16
+    block = (pix // W // 16) * bx + (pix % W) // 16
+    out = {"H_comp": int(gid.numel()), "R_hit": int((ncomp > 0).sum().item()),
+           "N_touched": int(torch.unique(gid).numel()), "P": int(torch.unique(block * n + gid).numel())}
+    del pix, gid, block, valid
+    return out
+
+
+def backward_bytes(c, R, n):
+    """Algorithmic bytes of one checkpoint-mode gt_backward call by kernel."""
+    b = {"k_zero_f32": 64 * n, "k_colors": 224 * n,
+         "k_backward": 4 * R + 16 * c["R_hit"] + 136 * c["H_comp"] + 112 * c["P"],
+         "k_finalize": 64 * n + 712 * c["N_touched"]}
+    b["total"] = sum(b.values())
+    return b
 
 
 def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd, st_fwd):
@@ -240,7 +428,28 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
     with MemSampler(torch) as mem_rc:
         ms_rc = _events_ms(torch, stream, step_rc, rc_steps)
     h_comp, h_eval = st_fwd["h_comp"], st_fwd["h_eval"]
-    B_fb = 16 * R + 44 * h_eval + 192 * h_comp + 16 * R + 44 * h_eval + (236 + 2 * 236) * h_comp
+    cnt = backward_counts(torch, cids, ncomp, W, H, n)
+    B = backward_bytes(cnt, R, n)
+    B_fwd_ck = 16 * R + 44 * h_eval + 192 * h_comp + 4 * h_comp + 4 * R  # + the checkpoint's ids and counts
+    B_loss = 80 * R + 236 * n  # dL/dRGBA = 2 (render - target) and zeroing the N x 59 gradient
+    B_step = B_fwd_ck + B_loss + B["total"]
+
+    def roof(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"algorithmic_bytes": int(nbytes), "ms": round(ms, 4), "achieved_gbs": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    roofline = {
+        "forward_with_checkpoint": roof(B_fwd_ck, ms_fwd_ck),
+        "backward": dict(roof(B["total"], ms_bwd), per_kernel_bytes={k: int(v) for k, v in B.items() if k != "total"}),
+        "step": dict(roof(B_step, ms_fb), note="step-level: forward with checkpoint + dL/dRGBA + gradient zeroing "
+                                                "+ backward, over the step's HIP-event time"),
+        "counts": cnt,
+        "bytes_model": "k_colors 224 N (16 B of mu + 192 B SH read, 16 B colour written); k_backward 4 R (n_comp) "
+                       "+ 16 R_hit (dL/dRGBA) + 136 H_comp (id twice, 48-B record and 16-B colour in each of two "
+                       "passes) + 112 P (one 56-B read-modify-write of the accumulators per distinct (16x16 block, "
+                       "Gaussian) pair); k_finalize 64 N (accumulators) + 712 N_touched (record 48, SH 192, "
+                       "236-B gradient row read and written); k_zero_f32 64 N",
+    }
     return {
         "workload": f"loss = sum (render - target)^2, target from means + N(0, 0.01); forward with checkpoint, "
                     f"dL/dRGBA, backward; {n} Gaussians, {W}x{H}, k {u.k}",
@@ -248,7 +457,7 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
         "ratio_vs_forward": round(ms_fb / ms_fwd, 3),
         "ms_forward_with_checkpoint": round(ms_fwd_ck, 3), "ms_backward": round(ms_bwd, 3),
         "checkpoint_cap": cap, "checkpoint_bytes": R * cap * 4 + R * 4, "max_composited_per_ray": most,
-        "algorithmic_bytes": int(B_fb), "roofline_frac": round(B_fb / (ms_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "roofline": roofline,
         "peak_device_bytes": int(mem.peak), "loss": loss,
         "grad_norms": {"mean": float(gn[0:3].norm()), "sh": float(gn[3:51].norm()), "logit": float(gn[51]),
                        "log_scale": float(gn[52:55].norm()), "quat": float(gn[55:59].norm())},
@@ -477,7 +686,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3, help="time 1/N of the rays on the CPU")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the launches whose HBM traffic roofline.traffic reports (for rocprofv3 --pmc)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic measurement")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.copy_calib:
         import torch
         a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -493,6 +708,14 @@ def main():
               "(rays would shard over GPUs by tile ranges with no exchange, each GPU holding the whole scene)",
               file=sys.stderr)
         sys.exit(2)
+
+    # HBM traffic first, by rocprofv3 child processes, before this process
+    # initialises the GPU
+    traffic, traffic_why = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        t_tr = time.time()
+        traffic, traffic_why = measure_traffic(args)
+        t_tr = time.time() - t_tr
 
     import numpy as np
     import torch
@@ -568,9 +791,20 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": (TRAFFIC_DEFAULT_FRAME["fetch_bytes"] + TRAFFIC_DEFAULT_FRAME["write_bytes"]
-                        if (args.n, W, H, K) == (1000000, 1920, 1080, 16) else None),
-            "algorithmic_bytes": int(B_fwd), "kernel": "k_forward<16, 0, false, false, false, true> (packet traversal)",
+            "traffic": (traffic["FETCH_SIZE"]["k_forward_bytes"] + traffic["WRITE_SIZE"]["k_forward_bytes"]
+                        if traffic else None),
+            "traffic_detail": ({"fetch_bytes": traffic["FETCH_SIZE"]["k_forward_bytes"],
+                                "write_bytes": traffic["WRITE_SIZE"]["k_forward_bytes"],
+                                "fetch_scale": traffic["FETCH_SIZE"]["scale"],
+                                "write_scale": traffic["WRITE_SIZE"]["scale"],
+                                "traffic_over_algorithmic": round((traffic["FETCH_SIZE"]["k_forward_bytes"] +
+                                                                   traffic["WRITE_SIZE"]["k_forward_bytes"]) / B_fwd, 4),
+                                "kernel_trace_ms": traffic["kernel_ms"].get("k_forward"),
+                                "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this script "
+                                          "(--traffic-probe: mean of the first two plain launches), scaled by the "
+                                          "counters' reading for a 1 GiB device copy (--copy-calib)",
+                                "measure_s": round(t_tr, 1)} if traffic else {"reason": traffic_why}),
+            "algorithmic_bytes": int(B_fwd), "kernel": FWD_KERNEL,
             "measured_copy_gbs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5),
         },
         "stats": st,
@@ -603,11 +837,25 @@ def main():
     if not args.no_backward:
         out["fwd_bwd"] = bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, max(2, args.steps // 2),
                                        min(args.warmup, 2), ms_step, st)
+        if traffic:
+            # the backward's kernels, each from its own trace duration and measured traffic
+            bb = out["fwd_bwd"]["roofline"]["backward"]["per_kernel_bytes"]
+            per = {}
+            for kname in ("k_colors", "k_backward", "k_finalize"):
+                d = traffic["kernel_ms"].get(kname) or []
+                ms_k = sum(d) / len(d) if d else None
+                gbs = bb[kname] / (ms_k * 1e-3) / 1e9 if ms_k else None
+                per[kname] = {"ms_trace": round(ms_k, 4) if ms_k else None, "algorithmic_bytes": bb[kname],
+                              "achieved_gbs": round(gbs, 1) if gbs else None,
+                              "frac": round(gbs / HBM_PEAK_GBS, 5) if gbs else None,
+                              "fetch_bytes": traffic["FETCH_SIZE"]["backward_bytes"][kname],
+                              "write_bytes": traffic["WRITE_SIZE"]["backward_bytes"][kname]}
+            out["fwd_bwd"]["roofline"]["backward"]["per_kernel"] = per
     if not args.no_train:
         out["device_build"] = bench_device_build(gt, torch, dev, sp, raw, u, W, H, max(2, args.steps // 2))
         out["device_build"]["host_build_ms"] = round(t_build * 1e3, 1)
     if not args.no_batched:
-        out["batched"] = bench_batched(gt, scene, torch, dev, sp, K, args.batch_cams)
+        out["batched"] = bench_batched(gt, raw, scene, torch, dev, sp, args.batch_cams)
     if not args.no_backward and not args.no_c4:
         out["fwd_bwd_c4"] = bench_c4(gt, torch, dev, sp, 5)
     if not args.no_backward and not args.no_train:
@@ -618,7 +866,7 @@ def main():
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
-        threads = min(16, os.cpu_count() or 1)
+        threads, nproc, model, quota, aff = cpu_info()
         o = orc.OracleScene(raw, 0.01)
         o.prebuild_bvh()
         p = orc.params_from_uniforms(u, W, H)
@@ -630,6 +878,7 @@ def main():
         cs = time.perf_counter() - c0
         out["cpu_baseline"] = {
             "value": round(n_s / cs / 1e6, 5), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "affinity_cpus": aff, "cgroup_cpu_quota": quota, "cpu_model": model,
             "sample": f"{n_s} seeded random rays (1/{args.cpu_sample} of the frame) of the same workload, "
                       f"oracle BVH variant (CPU restatement of the reference path, not the reference binary), "
                       f"{cs:.2f} s",

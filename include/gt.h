@@ -18,6 +18,10 @@
  * different threads.  Buffers are caller-owned.  With GT_BUF_DEVICE the
  * pointers are HIP device pointers and the call is asynchronous on the given
  * stream; otherwise they are host pointers and the call is synchronous.
+ * Renders of one scene may overlap on the GPU when issued on different
+ * streams: the persistent kernel keeps its queue counters per stream (a
+ * batch keeps its own), so only one batch replayed on two streams at once
+ * would share them.
  */
 #ifndef GT_H
 #define GT_H
@@ -181,14 +185,17 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
 /* Launch tuning of a scene (no counterpart in the reference; chosen per
  * scene by the autotuner).  Zero fields mean "library default".  None of
  * them changes a result: hit lists and pixels are identical for all values.
- *   path          0 default (= 3), 1 tile kernel (one ray tile per wave,
- *                 per-lane traversal), 2 persistent kernel with a global ray
- *                 queue, 3 tile kernel with wave-wide packet traversal (the
- *                 wave walks the BVH together from one stack of (node, lane mask))
- *   refill_min    persistent: idle lanes of a wave that trigger a refill (1..64)
- *   comp_min      persistent: lanes waiting to composite that trigger a
+ *   path          0 default (= 4), 1 tile kernel (one ray tile per wave,
+ *                 per-lane traversal), 2 persistent kernel whose lanes refill
+ *                 from a global ray queue (per-lane traversal), 3 tile kernel
+ *                 with wave-wide packet traversal (the wave walks the BVH
+ *                 together from one stack of (node, lane mask)), 4 persistent
+ *                 kernel with packet traversal: resident waves take 64-ray
+ *                 tiles from a global queue (one counter per XCD)
+ *   refill_min    path 2: idle lanes of a wave that trigger a refill (1..64)
+ *   comp_min      path 2: lanes waiting to composite that trigger a
  *                 composite pass (1..64)
- *   blocks_per_cu persistent: resident 256-thread blocks per CU (1..8)
+ *   blocks_per_cu paths 2 and 4: resident 256-thread blocks per CU (1..8)
  *   tile_w        ray tile of a wave: tile_w x (64 / tile_w) pixels, tile_w in {4, 8, 16, 32}
  *   stack_lds     traversal stack entries per ray kept in LDS (8, 16, 32 or 64);
  *                 deeper entries spill to scratch memory */
@@ -330,7 +337,7 @@ int gt_batch_render_noalloc(gt_scene* s, uint32_t n, const gt_uniforms* u, const
 typedef struct gt_autotune_result {
   gt_tuning best;
   double best_ms;   /* per render, best candidate */
-  double first_ms;  /* per render, first candidate (the default: packet traversal, 8x8 tiles) */
+  double first_ms;  /* per render, first candidate (the default: persistent packet kernel, 8x8 tiles) */
   uint32_t n_tried;
   uint32_t _pad;
 } gt_autotune_result;

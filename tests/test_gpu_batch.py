@@ -77,7 +77,7 @@ def _same(batch_out, single):
         assert b["rgba"].tobytes() == r["rgba"].tobytes(), f"camera {i}: pixels"
 
 
-@pytest.mark.parametrize("path", [1, 2, 3])
+@pytest.mark.parametrize("path", [1, 2, 3, 4])
 def test_batch_equals_single_camera(path):
     print(f"case batch path {path}", flush=True)
     s, c = _scene()
@@ -88,7 +88,7 @@ def test_batch_equals_single_camera(path):
     _same(out, single)
 
 
-@pytest.mark.parametrize("path", [1, 2, 3])
+@pytest.mark.parametrize("path", [1, 2, 3, 4])
 def test_batch_graph_replays(path):
     print(f"case batch graph path {path}", flush=True)
     s, c = _scene()
@@ -103,7 +103,8 @@ def test_batch_graph_replays(path):
 TUNINGS = [dict(path=1, tile_w=8), dict(path=1, tile_w=16), dict(path=1, tile_w=32, stack_lds=8),
            dict(path=1, tile_w=4, stack_lds=64), dict(path=2, refill_min=1, comp_min=1),
            dict(path=2, refill_min=64, comp_min=64, tile_w=16, stack_lds=16), dict(path=2, blocks_per_cu=1),
-           dict(path=3, tile_w=8), dict(path=3, tile_w=4), dict(path=3, tile_w=32)]
+           dict(path=3, tile_w=8), dict(path=3, tile_w=4), dict(path=3, tile_w=32),
+           dict(path=4, tile_w=8), dict(path=4, tile_w=4, blocks_per_cu=1), dict(path=4, tile_w=16, blocks_per_cu=8)]
 
 
 def test_tuning_never_changes_results():
@@ -228,3 +229,27 @@ def test_first_render_of_a_process_is_a_graph_batch():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "first-render graph batch ok" in r.stdout
+
+
+def test_one_scene_on_two_streams():
+    """Renders of one scene issued on two streams overlap on the GPU.  The
+    persistent kernel's queue counters are per stream, so every image equals
+    its render on the default stream alone, bit for bit."""
+    print("case one scene, two streams", flush=True)
+    s, c = _scene()
+    us = _cams(c, k=16)[:2]
+    sizes = SIZES[:2]
+    dev = torch.device("cuda:0")
+    ref = [s.render(u, w, h)["rgba"] for u, (w, h) in zip(us, sizes)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    imgs = [[torch.empty((h, w, 4), device=dev) for _ in range(6)] for (w, h) in sizes]
+    torch.cuda.synchronize()
+    for rep in range(6):
+        for i, (u, (w, h)) in enumerate(zip(us, sizes)):
+            with torch.cuda.stream(streams[i]):
+                s.render_device(u, w, h, imgs[i][rep].data_ptr(), streams[i].cuda_stream)
+    torch.cuda.synchronize()
+    for i in range(2):
+        for rep in range(6):
+            assert imgs[i][rep].cpu().numpy().tobytes() == ref[i].tobytes(), f"stream {i} render {rep}"
+    s.close()

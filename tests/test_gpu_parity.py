@@ -165,6 +165,30 @@ def test_packet_traversal_parity(k):
     check_parity(s, o, u, 80, 72, max_rounds=max(16, 512 // k), label=f"packet c1 k{k}")
 
 
+@pytest.mark.parametrize("k", [1, 16])
+def test_persistent_packet_kernel_parity(k):
+    """The persistent packet kernel (tuning path 4: resident waves take
+    64-ray tiles from per-XCD queues) against the oracle, with a grid small
+    enough that every wave renders many tiles and takes tiles from the other
+    XCDs' ranges."""
+    _print(f"case persistent packet kernel k{k}")
+    raw = gt.generate_raw(n=100000, seed=0, config="c2", n_clusters=8)
+    s, o = scene_pair(raw, 0.01)
+    c0 = raw[0::8, :3].mean(axis=0)
+    u = uniforms(list(c0 + np.array([0.3, 0.2, 1.5])), list(c0), 123, 77, hfov_deg=50, alpha_min=0.01, k=k,
+                 T_min=1e-3)
+    for bpc in (1, 4):
+        s.set_tuning(path=4, blocks_per_cu=bpc)
+        check_parity(s, o, u, 123, 77, max_rounds=max(16, 512 // k), label=f"persistent k{k} bpc{bpc}")
+    for mode in (1, 2):
+        u.debug_mode = mode
+        g = s.render(u, 123, 77, max_rounds=1)
+        import oracle as orc
+        r = orc.render(o, orc.params_from_uniforms(u, 123, 77), max_rounds=1)
+        np.testing.assert_array_equal(g["hit_ids"], r["hit_ids"])
+        np.testing.assert_allclose(g["rgba"].reshape(-1, 4), r["rgba"], atol=1e-5, rtol=0)
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_packet_traversal_debug_modes(mode):
     _print(f"case packet debug mode {mode}")

@@ -1,9 +1,10 @@
 /*
  * gt_autotune.cpp - per-scene choice of the launch tuning by measurement.
  *
- * Candidates differ only in how work is scheduled (tile kernel with
- * per-lane or wave-wide packet traversal, persistent ray queue, ray-tile
- * shape of a wave, LDS stack depth,
+ * Candidates differ only in how work is scheduled (wave-wide packet
+ * traversal in the persistent kernel or the tile kernel, per-lane traversal,
+ * the lane-refill persistent kernel, ray-tile shape of a wave, LDS stack
+ * depth,
  * persistent refill / composite thresholds); none changes a result.  Each
  * candidate renders the caller's workload (one frame or one batch) once to
  * warm up and then `iters` times between HIP events on the caller's stream;
@@ -20,7 +21,8 @@ namespace {
 
 std::vector<gt_tuning> candidates() {
   std::vector<gt_tuning> c;
-  for (uint32_t tw : {8u, 4u, 16u}) c.push_back(gt_tuning{3, 0, 0, 0, tw, 0});
+  for (uint32_t tw : {8u, 4u, 16u}) c.push_back(gt_tuning{4, 0, 0, 0, tw, 0}); /* first: the default */
+  for (uint32_t tw : {8u, 4u}) c.push_back(gt_tuning{3, 0, 0, 0, tw, 0});
   for (uint32_t tw : {8u, 16u, 32u})
     for (uint32_t lds : {16u, 32u}) c.push_back(gt_tuning{1, 0, 0, 0, tw, lds});
   for (uint32_t tw : {8u, 16u})

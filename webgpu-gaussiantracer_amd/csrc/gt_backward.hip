@@ -554,7 +554,7 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
     const uint32_t y0 = ty0 * 8u, y1 = ty1 * 8u < height ? ty1 * 8u : height;
     const uint32_t npix = (y1 - y0) * width;
     f.comp_base = y0 * width;
-    rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x);
+    rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x, nullptr);
     if (rc != GT_OK) break;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dmax);
     hipLaunchKernelGGL(k_max_u32, dim3((npix + 255) / 256 < 1024 ? (npix + 255) / 256 : 1024), dim3(256), 0, st,

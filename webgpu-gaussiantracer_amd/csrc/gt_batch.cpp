@@ -23,7 +23,6 @@
 /* gt_scene.cpp */
 const gt_dev_scene* gt_scene_dev(const gt_scene* s);
 const gt_tuning* gt_scene_tuning(const gt_scene* s);
-unsigned int* gt_scene_work(gt_scene* s);
 int gt_validate_uniforms(const gt_uniforms* u, uint32_t width, uint32_t height);
 void gt_fill_dev_frame(const gt_uniforms* u, uint32_t width, uint32_t height, gt_dev_frame* f);
 uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base);
@@ -33,6 +32,7 @@ struct gt_batch {
   std::vector<gt_dev_frame> frames;
   void* table = nullptr;
   size_t table_bytes = 0;
+  void* work = nullptr; /* the persistent renderers' work counters (GT_WORK_WORDS), this batch's own */
   uint32_t total_tiles = 0;
   gt_tuning tune_used{};
   bool table_valid = false;
@@ -45,7 +45,6 @@ struct gt_batch {
    * that falls back to one), after which the graph would read freed memory;
    * the graph is recaptured whenever these differ from the scene's. */
   gt_dev_scene dev_used{};
-  unsigned int* work_used = nullptr;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -57,6 +56,7 @@ struct gt_batch {
     drop_graph();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (table) (void)hipFree(table);
+    if (work) (void)hipFree(work);
   }
 };
 
@@ -64,9 +64,7 @@ struct gt_batch {
  * table; drops a captured graph whose launch arguments are out of date */
 static int prepare_table(gt_batch* b) {
   const gt_tuning* t = gt_scene_tuning(b->scene);
-  if (b->exec && (std::memcmp(gt_scene_dev(b->scene), &b->dev_used, sizeof(gt_dev_scene)) != 0 ||
-                  gt_scene_work(b->scene) != b->work_used))
-    b->drop_graph();
+  if (b->exec && std::memcmp(gt_scene_dev(b->scene), &b->dev_used, sizeof(gt_dev_scene)) != 0) b->drop_graph();
   if (b->table_valid && std::memcmp(t, &b->tune_used, sizeof(gt_tuning)) == 0) return GT_OK;
   uint64_t base = 0;
   for (auto& f : b->frames) {
@@ -103,10 +101,11 @@ static int launch(gt_batch* b, hipStream_t st) {
   const gt_tuning* t = gt_scene_tuning(b->scene);
   const gt_dev_frame* cams = (const gt_dev_frame*)b->table;
   const uint32_t n = (uint32_t)b->frames.size();
+  unsigned int* work = (unsigned int*)b->work;
   if (t->path == 2)
-    return gt_launch_persistent(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles,
-                                gt_scene_work(b->scene), (unsigned long long*)b->frames[0].stats, t, st);
-  return gt_launch_forward(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, t, st, 0);
+    return gt_launch_persistent(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, work,
+                                (unsigned long long*)b->frames[0].stats, t, st);
+  return gt_launch_forward(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, t, st, 0, work);
 }
 
 extern "C" int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, const uint32_t* widths,
@@ -123,6 +122,12 @@ extern "C" int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, co
   gt_batch* b = new (std::nothrow) gt_batch();
   if (!b) { gt_set_error("out of host memory"); return GT_E_NOMEM; }
   b->scene = s;
+  if (hipMalloc(&b->work, GT_WORK_WORDS * sizeof(unsigned int)) != hipSuccess) {
+    b->work = nullptr;
+    delete b;
+    gt_set_error("hipMalloc of the batch's work counters failed");
+    return GT_E_NOMEM;
+  }
   b->frames.resize(n);
   const bool dump = outs[0].hit_ids != nullptr;
   for (uint32_t i = 0; i < n; ++i) {
@@ -202,7 +207,6 @@ extern "C" int gt_batch_render(gt_batch* b, int use_graph, void* stream) {
       return GT_E_HIP;
     }
     b->dev_used = *gt_scene_dev(b->scene);
-    b->work_used = gt_scene_work(b->scene);
   }
   hipError_t e = hipGraphLaunch(b->exec, st);
   if (e != hipSuccess) {

@@ -123,7 +123,7 @@ struct KBuf {
 };
 
 struct Counters {
-  uint32_t rounds = 0, h_eval = 0, h_comp = 0, proxies = 0, hits = 0, nodes = 0, overflow = 0;
+  uint32_t rays = 0, rounds = 0, h_eval = 0, h_comp = 0, proxies = 0, hits = 0, nodes = 0, overflow = 0;
 };
 
 enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
@@ -415,28 +415,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-/* One wave = one tile of tile_w x (64 / tile_w) pixels; block ids are
- * remapped so that consecutive tiles share an XCD.  BATCH: global tiles of
- * all cameras are numbered consecutively (camera c owns tiles from
- * cams[c].tile_base). */
-template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH, bool PACKET>
-__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
-                                                                    const gt_dev_frame* __restrict__ cams,
-                                                                    uint32_t n_cams, uint32_t total_tiles,
-                                                                    uint32_t tile_w, uint32_t lds_depth,
-                                                                    uint32_t tile_offset) {
-  extern __shared__ uint32_t s_stack[];
-  __shared__ float s_faces[12 * GT_NUM_FACES];
-  for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
-  __syncthreads();
-  Stack stk;
-  stk.lds = s_stack + threadIdx.x;
-  stk.sp = 0;
-  uint32_t* const wst = s_stack + (threadIdx.x >> 6) * (3u * GT_STACK_MAX); /* PACKET: this wave's stack */
-  stk.depth = (int)lds_depth;
-  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t tile = tile_offset + blk * 4u + (threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63u;
+/* One 64-ray tile of tile_w x (64 / tile_w) pixels, rendered by one wave
+ * (GS_shader.ts:275-309 per lane).  BATCH: global tiles of all cameras are
+ * numbered consecutively (camera c owns tiles from cams[c].tile_base). */
+template <int MAXK, int MODE, bool DUMP, bool BATCH, bool PACKET>
+__device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_frame& F0,
+                                            const gt_dev_frame* __restrict__ cams, uint32_t n_cams,
+                                            uint32_t total_tiles, uint32_t tile_w, uint32_t tile, uint32_t lane,
+                                            Stack& stk, uint32_t* wst, const float* s_faces, Counters& st) {
   const uint32_t cam = (BATCH && tile < total_tiles) ? find_camera(cams, n_cams, tile) : 0u;
   const gt_dev_frame& F = BATCH ? cams[cam] : F0;
   const uint32_t tile_h = 64u / tile_w;
@@ -444,7 +430,6 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
   const uint32_t px = (lt % F.tiles_x) * tile_w + (lane % tile_w);
   const uint32_t py = (lt / F.tiles_x) * tile_h + (lane / tile_w);
   const bool active = tile < total_tiles && px < F.width && py < F.height;
-  Counters st;
   if (active) {
     const uint32_t ray = py * F.width + px;
     RayState r;
@@ -575,14 +560,63 @@ __global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev
       if (MODE != MODE_RENDER && MR > 0)
         for (uint32_t j = 1; j < kk; ++j) F.hit_ids[(size_t)ray * MR * kk + j] = GT_SENTINEL;
     }
-    st.rounds = rounds;
+    st.rounds += rounds;
+    st.rays++;
+  }
+}
+
+/* The tile kernel: one wave per tile, block ids remapped so that
+ * consecutive tiles share an XCD (and its L2).  PERSIST: a grid sized to
+ * the machine whose waves claim tiles from a global queue instead, one
+ * counter per XCD over a contiguous eighth of the tiles (a wave whose XCD
+ * range is exhausted takes tiles from the other ranges), so a launch over
+ * many cameras of uneven size keeps every CU busy to its end.  Each tile
+ * is rendered by the same code either way. */
+template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH, bool PACKET, bool PERSIST>
+__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
+                                                                    const gt_dev_frame* __restrict__ cams,
+                                                                    uint32_t n_cams, uint32_t total_tiles,
+                                                                    uint32_t tile_w, uint32_t lds_depth,
+                                                                    uint32_t tile_offset, unsigned int* work) {
+  extern __shared__ uint32_t s_stack[];
+  __shared__ float s_faces[12 * GT_NUM_FACES];
+  for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
+  __syncthreads();
+  Stack stk;
+  stk.lds = s_stack + threadIdx.x;
+  stk.sp = 0;
+  uint32_t* const wst = s_stack + (threadIdx.x >> 6) * (3u * GT_STACK_MAX); /* PACKET: this wave's stack */
+  stk.depth = (int)lds_depth;
+  const uint32_t lane = threadIdx.x & 63u;
+  Counters st;
+  if (!PERSIST) {
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = tile_offset + blk * 4u + (threadIdx.x >> 6);
+    render_tile<MAXK, MODE, DUMP, BATCH, PACKET>(S, F0, cams, n_cams, total_tiles, tile_w, tile, lane, stk, wst,
+                                                 s_faces, st);
+  } else {
+    const uint32_t n = total_tiles - tile_offset;
+    const uint32_t x0 = blockIdx.x & (GT_XCDS - 1u);
+    for (uint32_t j = 0; j < GT_XCDS; ++j) {
+      const uint32_t x = (x0 + j) & (GT_XCDS - 1u);
+      const uint32_t lo = tile_offset + (uint32_t)((uint64_t)n * x / GT_XCDS);
+      const uint32_t hi = tile_offset + (uint32_t)((uint64_t)n * (x + 1u) / GT_XCDS);
+      for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(work + GT_WORK_STRIDE * x, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= hi - lo) break; /* every wave of the grid reaches this exit once the ranges are claimed */
+        render_tile<MAXK, MODE, DUMP, BATCH, PACKET>(S, F0, cams, n_cams, total_tiles, tile_w, lo + t, lane, stk,
+                                                     wst, s_faces, st);
+      }
+    }
   }
   if (STATS) {
-    const uint32_t v[8] = {active ? 1u : 0u, st.rounds, st.h_eval, st.h_comp, st.proxies, st.hits, st.nodes, st.overflow};
+    const uint32_t v[8] = {st.rays, st.rounds, st.h_eval, st.h_comp, st.proxies, st.hits, st.nodes, st.overflow};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const uint32_t s = wave_sum(v[i]);
-      if (lane == 0 && s) atomicAdd(F.stats + i, (unsigned long long)s);
+      const uint32_t sum = wave_sum(v[i]);
+      if (lane == 0 && sum) atomicAdd(F0.stats + i, (unsigned long long)sum);
     }
   }
 }
@@ -868,30 +902,52 @@ __global__ __launch_bounds__(256) void k_colors(const gt_dev_scene S, float ox, 
   out[i] = make_float4(c.x, c.y, c.z, 0.0f);
 }
 
-template <int MAXK, int MODE, bool BATCH, bool PACKET>
+__global__ void k_reset_work(unsigned int* work) {
+  if (threadIdx.x < GT_WORK_WORDS) work[threadIdx.x] = 0u;
+}
+
+int device_cus();
+
+template <int MAXK, int MODE, bool BATCH, bool PACKET, bool PERSIST>
 hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                           uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                          bool stats, uint32_t tile_offset) {
-  const dim3 grid((total_tiles - tile_offset + 3u) / 4u), block(GT_BLOCK);
+                          bool stats, uint32_t tile_offset, unsigned int* work, uint32_t bpc) {
+  dim3 grid((total_tiles - tile_offset + 3u) / 4u);
+  if (PERSIST) {
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(GT_WORK_WORDS), 0, st, work);
+    const uint32_t g = (uint32_t)device_cus() * bpc;
+    grid = dim3(g < grid.x ? g : grid.x);
+  }
+  const dim3 block(GT_BLOCK);
   const size_t lds = PACKET ? (size_t)(GT_BLOCK / 64) * 3u * GT_STACK_MAX * sizeof(uint32_t)
                             : (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
-#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset
-  if (dump && stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
-  else if (dump) hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
-  else if (stats) hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
-  else hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false, BATCH, PACKET>), grid, block, lds, st, GT_ARGS);
+#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset, work
+  if (dump && stats)
+    hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH, PACKET, PERSIST>), grid, block, lds, st, GT_ARGS);
+  else if (dump)
+    hipLaunchKernelGGL((k_forward<MAXK, MODE, true, false, BATCH, PACKET, PERSIST>), grid, block, lds, st, GT_ARGS);
+  else if (stats)
+    hipLaunchKernelGGL((k_forward<MAXK, MODE, false, true, BATCH, PACKET, PERSIST>), grid, block, lds, st, GT_ARGS);
+  else
+    hipLaunchKernelGGL((k_forward<MAXK, MODE, false, false, BATCH, PACKET, PERSIST>), grid, block, lds, st, GT_ARGS);
 #undef GT_ARGS
   return hipGetLastError();
 }
 
+/* path 1: per-lane traversal; 3 (default): packet traversal; 4: packet
+ * traversal in the persistent kernel */
 template <int MAXK, int MODE, bool BATCH>
 hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                         uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                        bool stats, uint32_t tile_offset, bool packet) {
-  return packet ? launch_tiles_p<MAXK, MODE, BATCH, true>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
-                                                          dump, stats, tile_offset)
-                : launch_tiles_p<MAXK, MODE, BATCH, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
-                                                           dump, stats, tile_offset);
+                        bool stats, uint32_t tile_offset, uint32_t path, unsigned int* work, uint32_t bpc) {
+  if (path == 1)
+    return launch_tiles_p<MAXK, MODE, BATCH, false, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
+                                                           dump, stats, tile_offset, work, bpc);
+  if (path == 4)
+    return launch_tiles_p<MAXK, MODE, BATCH, true, true>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
+                                                         dump, stats, tile_offset, work, bpc);
+  return launch_tiles_p<MAXK, MODE, BATCH, true, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st, dump,
+                                                        stats, tile_offset, work, bpc);
 }
 
 template <int MAXK, bool BATCH>
@@ -994,7 +1050,7 @@ extern "C" int gt_launch_colors(const gt_dev_scene* S, const gt_dev_frame* F, fl
 
 extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, const gt_dev_frame* cams_dev,
                                  uint32_t n_cams, uint32_t total_tiles, const gt_tuning* tune, void* stream,
-                                 uint32_t tile_offset) {
+                                 uint32_t tile_offset, unsigned int* work) {
   if (tile_offset >= total_tiles) return GT_OK;
   hipError_t e = ensure_faces();
   if (e != hipSuccess) {
@@ -1008,22 +1064,30 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   const bool dump = F->hit_ids != nullptr && F->max_rounds > 0;
   const bool stats = F->stats != nullptr;
   const bool batch = n_cams > 1;
-  /* default: packet traversal (path 0 or 3); path 1: per-lane traversal */
-  const bool packet = !tune || tune->path == 0 || tune->path == 3;
+  /* default (path 0): packet traversal in the persistent kernel when work
+   * counters are given, else in the tile kernel; path 1: per-lane
+   * traversal; 3: packet traversal, tile kernel; 4: packet traversal,
+   * persistent kernel */
+  const uint32_t path = (!tune || tune->path == 0) ? (work ? 4u : 3u) : tune->path;
+  if (path == 4 && !work) {
+    gt_set_error("the persistent packet kernel (tuning path 4) needs its work counters");
+    return GT_E_INVALID;
+  }
+  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_MIN_WAVES;
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
 #define GT_TILES(MK, MODE)                                                                                       \
   (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                       tile_offset, packet)                                                      \
+                                       tile_offset, path, work, bpc)                                             \
          : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                        tile_offset, packet))
+                                        tile_offset, path, work, bpc))
   if (F->debug_mode == 1) {
     e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                             tile_offset, packet);
+                                             tile_offset, path, work, bpc);
   } else if (F->debug_mode == 2) {
     e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                           tile_offset, packet);
+                                           tile_offset, path, work, bpc);
   } else if (F->k <= 4) {
     e = GT_TILES(4, MODE_RENDER);
   } else if (F->k <= 8) {

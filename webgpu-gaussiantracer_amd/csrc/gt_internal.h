@@ -31,6 +31,11 @@
 #define GT_LEAF_FIRST_MASK ((1u << GT_LEAF_COUNT_SHIFT) - 1u)
 #define GT_MAX_PROXIES GT_LEAF_FIRST_MASK
 #define GT_STACK_MAX 96               /* traversal stack entries per ray */
+/* Work counters of the persistent renderers: one per XCD (gfx950 has 8),
+ * GT_WORK_STRIDE words apart so each sits in its own 64-B line. */
+#define GT_XCDS 8u
+#define GT_WORK_STRIDE 16u
+#define GT_WORK_WORDS (GT_XCDS * GT_WORK_STRIDE)
 /* SAH cost of a proxy test relative to a 4-wide node visit (host and
  * device builders).  0.5 measured fastest at 1M Gaussians, 1080p, k 16:
  * 14.6 ms per frame against 15.8 ms at 2.0 (host tree) and 15.4 against
@@ -99,9 +104,13 @@ extern "C" {
 /* implemented in gt_forward.hip */
 /* Tile renderer over n_cams cameras (frame F used directly when n_cams == 1,
  * else the device array cams_dev with tile_base/tiles_x set for tune's
- * tile shape); total_tiles = all cameras' tiles. */
+ * tile shape); total_tiles = all cameras' tiles.  Tuning path 4 renders the
+ * same tiles with the persistent packet kernel, whose queue counters are
+ * `work` (GT_WORK_WORDS device words, reset by the launch; required for
+ * path 4, unused otherwise). */
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
-                      uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset);
+                      uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,
+                      unsigned int* work);
 /* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
  * array of n_cams frames (tile_base set, ascending); cam0: host copy of
  * frame 0, used directly when n_cams == 1.  work: one device counter the

@@ -5,7 +5,7 @@
  * For callers that cannot call hipMalloc: code under HIP graph capture,
  * tight serving loops, harnesses that forbid it.  The batch path's scratch
  * is the camera table the render kernels read (one gt_dev_frame per camera,
- * with its tile numbering) and the persistent renderer's work counter.
+ * with its tile numbering) and the persistent renderers' work counters.
  * Here both live in `scratch`, a file-scope device array of fixed size,
  * which every GPU the library runs on holds until the process ends.  Host
  * code never passes `scratch` itself to a kernel or a copy (in host code it
@@ -33,7 +33,7 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
 
 namespace {
 
-constexpr size_t kCounterBytes = 256; /* the work counter, padded to keep the table 256-B aligned */
+constexpr size_t kCounterBytes = 512; /* the work counters (GT_WORK_WORDS), padded to keep the table 256-B aligned */
 constexpr size_t kScratchBytes = kCounterBytes + (size_t)GT_NOALLOC_MAX_CAMERAS * sizeof(gt_dev_frame);
 constexpr uint32_t kChunk = 16; /* cameras per table-writing launch (16 x 208 B of kernel arguments) */
 
@@ -163,5 +163,5 @@ extern "C" int gt_batch_render_noalloc(gt_scene* s, uint32_t n, const gt_uniform
   if (t->path == 2)
     return gt_launch_persistent(gt_scene_dev(s), &first, table, n, total, work,
                                 (unsigned long long*)outs[0].stats, t, stream);
-  return gt_launch_forward(gt_scene_dev(s), &first, table, n, total, t, stream, 0);
+  return gt_launch_forward(gt_scene_dev(s), &first, table, n, total, t, stream, 0, work);
 }

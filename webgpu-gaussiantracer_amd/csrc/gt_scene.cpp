@@ -616,9 +616,9 @@ int gt_scene_static_upload(gt_scene* s) {
     face_table(faces);
     if ((rc = upload(s->faces, faces, 12 * GT_NUM_FACES)) != GT_OK) return rc;
   }
-  if (!s->ctl.p) {
-    const uint32_t zero[64] = {0};
-    if ((rc = upload(s->ctl, zero, 64)) != GT_OK) return rc;
+  if (!s->ctl.p) { /* the persistent renderers' work counters, one set per stream */
+    const std::vector<uint32_t> zero((size_t)GT_WORK_SLOTS * GT_WORK_WORDS, 0u);
+    if ((rc = upload(s->ctl, zero.data(), zero.size())) != GT_OK) return rc;
   }
   return GT_OK;
 }
@@ -955,13 +955,37 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
   return f->tiles_x * ((f->height + th - 1u) / th);
 }
 
+/* The work counters of `stream` in this scene (assigned on the stream's
+ * first render), or NULL once GT_WORK_SLOTS streams hold one. */
+unsigned int* gt_scene_work(gt_scene* s, void* stream) {
+  std::lock_guard<std::mutex> lk(s->ctl_mu);
+  unsigned int* base = (unsigned int*)s->ctl.p;
+  if (!base) return nullptr;
+  for (uint32_t i = 0; i < s->ctl_used; ++i)
+    if (s->ctl_streams[i] == stream) return base + (size_t)i * GT_WORK_WORDS;
+  if (s->ctl_used == GT_WORK_SLOTS) return nullptr;
+  s->ctl_streams[s->ctl_used] = stream;
+  return base + (size_t)(s->ctl_used++) * GT_WORK_WORDS;
+}
+
+/* The persistent packet kernel by default (tuning path 0 or 4); the tile
+ * kernels with paths 1 and 3; the lane-refill persistent kernel with path 2
+ * (which leaves debug modes, the backward checkpoint and antialiasing to the
+ * tile kernel).  A persistent kernel needs this stream's work counters; a
+ * scene rendered from more than GT_WORK_SLOTS streams renders on the
+ * further streams with the tile kernel (results never depend on the path). */
 static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   const uint32_t tiles = gt_frame_tiles(&s->tune, f, 0);
-  /* the checkpoint for the backward pass and antialiasing are the tile kernel's */
-  if (f->debug_mode != 0 || s->tune.path != 2 || f->comp_ids || f->n_comp || f->antialias)
-    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &s->tune, stream, 0);
-  return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, (unsigned int*)s->ctl.p, (unsigned long long*)f->stats,
-                              &s->tune, stream);
+  const uint32_t path = s->tune.path == 0 ? 4u : s->tune.path;
+  unsigned int* work = (path == 2 || path == 4) ? gt_scene_work(s, stream) : nullptr;
+  gt_tuning t = s->tune;
+  t.path = path;
+  if ((path == 2 || path == 4) && !work) t.path = 3;
+  if (t.path != 2 || f->debug_mode != 0 || f->comp_ids || f->n_comp || f->antialias) {
+    if (t.path == 2) t.path = 3;
+    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work);
+  }
+  return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, work, (unsigned long long*)f->stats, &t, stream);
 }
 
 extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
@@ -969,7 +993,7 @@ extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
   const bool tile_ok = t->tile_w == 0 || t->tile_w == 4 || t->tile_w == 8 || t->tile_w == 16 || t->tile_w == 32;
   const bool lds_ok = t->stack_lds == 0 || t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32 ||
                       t->stack_lds == 64;
-  if (t->path > 3 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
+  if (t->path > 4 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
     gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u, tile_w %u, "
                  "stack_lds %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w, t->stack_lds);
     return GT_E_INVALID;
@@ -986,7 +1010,6 @@ extern "C" int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t) {
 
 const gt_dev_scene* gt_scene_dev(const gt_scene* s) { return &s->dev; }
 const gt_tuning* gt_scene_tuning(const gt_scene* s) { return &s->tune; }
-unsigned int* gt_scene_work(gt_scene* s) { return (unsigned int*)s->ctl.p; }
 
 static void* grow(DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes) return b.p;

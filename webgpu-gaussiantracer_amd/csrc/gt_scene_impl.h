@@ -8,9 +8,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 
 #include "gt_internal.h"
+
+#define GT_WORK_SLOTS 32u
 
 struct HostScene {
   uint32_t n = 0;
@@ -45,7 +48,13 @@ struct gt_scene {
   DevBuf geom, sh, prox, nodes, faces;
   DevBuf raw_dev; /* device-built scenes: copy of the n x 59 PLY-raw parameters */
   gt_devbuild* db = nullptr;
-  DevBuf ctl; /* work counter of the persistent renderer */
+  /* work counters of the persistent renderers: one set of GT_WORK_WORDS per
+   * stream that rendered the scene (up to GT_WORK_SLOTS streams), so renders
+   * of one scene on different streams never share a queue */
+  DevBuf ctl;
+  void* ctl_streams[GT_WORK_SLOTS] = {};
+  uint32_t ctl_used = 0;
+  std::mutex ctl_mu;
   DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
   DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
   DevBuf aa;  /* antialiasing: the new frame before it is averaged in */

@@ -525,6 +525,69 @@ def autotune_batch(scene, batch, use_graph=True, iters=3, stream_ptr=None):
     return _autotune_dict(r)
 
 
+def autotune_batch_k(scene, uniforms, sizes, ks=(4, 8, 16, 32), tile_ws=(4, 8, 16, 32), iters=2,
+                     stream_ptr=None):
+    """Opt-in choice of hit_array_size for a batch, with the ray-tile shape.
+
+    k changes the image (round boundaries move the T_min checks and the 1e-4
+    relaunch window), so it is the caller's to accept: for each k the
+    cameras are rendered one gt_render at a time (the plain kernel at that
+    k) and as one batch under a hipGraph for each tile shape of the
+    persistent packet kernel; the batch must equal the plain renders bit
+    for bit, and the fastest tile shape is kept.  Returns one dict per k
+    (ms per batch, the plain launches' ms, tile_w, equal) and the fastest k.
+    Uses torch for the device buffers; leaves the scene's tuning as found."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    stream = torch.cuda.current_stream()
+    saved = scene.get_tuning()
+    out = []
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    try:
+        for k in ks:
+            us = []
+            for u in uniforms:
+                v = Uniforms.from_buffer_copy(u)
+                v.k = k
+                us.append(v)
+            plain = [torch.empty((h, w, 4), dtype=torch.float32, device=dev) for w, h in sizes]
+            bufs = [torch.empty((h, w, 4), dtype=torch.float32, device=dev) for w, h in sizes]
+            scene.set_tuning(**saved)
+
+            def render_plain():
+                for v, (w, h), b in zip(us, sizes, plain):
+                    scene.render_device(v, w, h, b.data_ptr(), stream_ptr)
+            render_plain()
+            plain_ms = timed(render_plain, iters)
+            best = None
+            for tw in tile_ws:
+                scene.set_tuning(path=4, tile_w=tw)
+                b = Batch(scene, us, sizes, [dict(rgba=x.data_ptr()) for x in bufs])
+                b.render(stream_ptr, use_graph=True)  # capture
+                ms = timed(lambda: b.render(stream_ptr, use_graph=True), iters)
+                equal = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(bufs, plain))
+                b.close()
+                if not equal:
+                    raise GTError(GT_OK - 1, f"batch at k={k} tile_w={tw} differs from the plain renders")
+                if best is None or ms < best[0]:
+                    best = (ms, tw)
+            out.append(dict(k=k, ms=best[0], tile_w=best[1], plain_ms=plain_ms, equal_to_plain=True))
+            del plain, bufs
+    finally:
+        scene.set_tuning(**saved)
+    fastest = min(out, key=lambda r: r["ms"])["k"]
+    return out, fastest
+
+
 def backward(scene, u, width, height, dl_drgba_ptr, grads_ptr, comp_ids_ptr=None, n_comp_ptr=None, comp_cap=0,
              overflow_ptr=None, budget_bytes=0, stream_ptr=None):
     """Accumulates dLoss/draw (n x 59, device) for one frame; device pointers."""
