@@ -237,7 +237,7 @@ def cameras_c3(n, seed=1):
     return [list(25.0 * d) for d in dirs], sizes
 
 
-def bench_batched(gt, raw, scene, torch, dev, sp, n_cams):
+def bench_batched(gt, raw, scene, torch, dev, sp, n_cams, alphas=(0.01, 0.1), ks=(4, 8, 16, 32)):
     """Batched forward: n_cams cameras of uneven resolution rendered by one
     launch replayed from a hipGraph, for alpha_min 0.01 and 0.1 and
     hit_array_size 4, 8, 16, 32; for each, the ray-tile shape (4, 8, 16 or
@@ -248,12 +248,11 @@ def bench_batched(gt, raw, scene, torch, dev, sp, n_cams):
     poses, sizes = cameras_c3(n_cams)
     rays = sum(w * h for w, h in sizes)
     sweep, fastest = [], {}
-    for alpha in (0.01, 0.1):
+    for alpha in alphas:
         sc = scene if alpha == 0.01 else gt.Scene(raw, alpha_min=alpha)
         us = [gt.make_uniforms(gt.camera_look_at(p, [0.0, 0.0, 0.0]), w, h, hfov_deg=50.0, T_min=1e-3,
                                alpha_min=alpha, k=16) for p, (w, h) in zip(poses, sizes)]
-        res, fk = gt.autotune_batch_k(sc, us, sizes, ks=(4, 8, 16, 32), tile_ws=(4, 8, 16, 32), iters=2,
-                                      stream_ptr=sp)
+        res, fk = gt.autotune_batch_k(sc, us, sizes, ks=ks, tile_ws=(4, 8, 16, 32), iters=2, stream_ptr=sp)
         fastest[str(alpha)] = fk
         for r in res:
             sweep.append({"alpha_min": alpha, "k": r["k"], "mrays": round(rays / r["ms"] / 1e3, 3),
@@ -264,7 +263,7 @@ def bench_batched(gt, raw, scene, torch, dev, sp, n_cams):
                           "equal_to_plain": r["equal_to_plain"]})
         if sc is not scene:
             sc.close()
-    head = [r for r in sweep if r["alpha_min"] == 0.01 and r["k"] == 16][0]
+    head = ([r for r in sweep if r["alpha_min"] == 0.01 and r["k"] == 16] or sweep)[0]
     return {
         "workload": f"{n_cams} cameras on a sphere of radius 25, resolutions uniform from {BATCH_SIZES}, hFov 50, "
                     f"T_min 0.001; one launch per frame replayed from a hipGraph (persistent packet kernel, ray "
@@ -679,6 +678,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-camera measurement")
     ap.add_argument("--batch-cams", type=int, default=64)
+    ap.add_argument("--batch-alphas", type=float, nargs="+", default=[0.01, 0.1],
+                    help="alpha_min values of the batched sweep")
+    ap.add_argument("--batch-ks", type=int, nargs="+", default=[4, 8, 16, 32], help="k values of the batched sweep")
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the 3M-Gaussian 16-view forward+backward")
     ap.add_argument("--no-c4", action="store_true", help="skip the 200k-Gaussian 800x800 forward+backward")
@@ -855,7 +857,8 @@ def main():
         out["device_build"] = bench_device_build(gt, torch, dev, sp, raw, u, W, H, max(2, args.steps // 2))
         out["device_build"]["host_build_ms"] = round(t_build * 1e3, 1)
     if not args.no_batched:
-        out["batched"] = bench_batched(gt, raw, scene, torch, dev, sp, args.batch_cams)
+        out["batched"] = bench_batched(gt, raw, scene, torch, dev, sp, args.batch_cams, tuple(args.batch_alphas),
+                                       tuple(args.batch_ks))
     if not args.no_backward and not args.no_c4:
         out["fwd_bwd_c4"] = bench_c4(gt, torch, dev, sp, 5)
     if not args.no_backward and not args.no_train:
