@@ -86,10 +86,14 @@ void gt_camera_look_at(const float pos[3], const float target[3], float to_world
  * quaternion of zero norm, or a scale exp(log-scale) that is 0 or inf in
  * f32 (for packed records: a non-finite field, a scale <= 0 or a zero
  * quaternion) is GT_E_INVALID, and gt_last_error() names the first such
- * Gaussian.  The reference has no check (GS_ply.ts:225-235 divides by the
- * quaternion norm and builds NaN proxies).  After such an error from
- * gt_scene_update_device / gt_scene_refit_device the scene's contents are
- * unspecified until an update with valid parameters succeeds. */
+ * Gaussian.  So is a Gaussian with a proxy (opacity > alpha_min) whose
+ * proxy transform, its inverse, its world box or the box centre is not
+ * finite in f32: a position near the f32 limit, a scale whose product with
+ * sqrt(2 ln(opacity / alpha_min)) overflows, or one so small that the
+ * inverse overflows.  The reference has no check (GS_ply.ts:225-235
+ * divides by the quaternion norm and builds NaN proxies).  After such an
+ * error from gt_scene_update_device / gt_scene_refit_device the scene's
+ * contents are unspecified until an update with valid parameters succeeds. */
 
 /* From PLY-raw parameters (n x 59 floats): applies the activations of
  * src/GS_ply.ts:225-235 in double, packs 320-B records, filters proxies by

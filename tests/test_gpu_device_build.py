@@ -173,3 +173,52 @@ def test_host_input_gpu_build():
     x = torch.from_numpy(raw).to("cuda:0")
     g.update_device(x.data_ptr())  # a scene from host input can be updated from device parameters
     _same_render(h, g, u, 96, 80, label="raw_gpu updated")
+
+
+def _bad_raws():
+    base = gt.generate_raw(n=4000, seed=3, config="c1")
+    base[1737, 51] = 3.0  # opacity 0.95: a proxy at alpha_min 0.05
+    cases = []
+    for label, col, val in [("nan mean", 1, np.nan), ("scale overflows f32", 52, 100.0),
+                            ("position near FLT_MAX", 0, 3.0e38), ("scale x proxy factor overflows", 53, 88.5),
+                            ("inverse overflows", 52, -87.0)]:
+        r = base.copy()
+        r[1737, col] = val
+        cases.append((label, r))
+    r = base.copy()
+    r[1737, 55:59] = 0.0
+    cases.append(("zero quaternion", r))
+    return cases
+
+
+@pytest.mark.parametrize("label,raw", _bad_raws(), ids=[c[0] for c in _bad_raws()])
+def test_invalid_records_rejected_by_both_builders(label, raw):
+    """The host and the GPU builder refuse the same records (non-finite
+    parameters, zero quaternion, scales 0 or inf in f32, proxies whose
+    transform, inverse or box overflows f32) with GT_E_INVALID naming the
+    Gaussian; gt_scene_update_device and gt_scene_refit_device refuse them
+    for a valid scene, which renders as before once updated with valid
+    parameters again."""
+    _print("invalid record:", label)
+    with pytest.raises(gt.GTError) as e:
+        gt.Scene(raw=raw, alpha_min=0.05)
+    assert e.value.code == -1 and "Gaussian 1737" in str(e.value), str(e.value)
+    t = torch.from_numpy(np.ascontiguousarray(raw, np.float32)).to("cuda:0")
+    with pytest.raises(gt.GTError) as e:
+        gt.Scene(raw_dev=t.data_ptr(), n=raw.shape[0], alpha_min=0.05)
+    assert e.value.code == -1 and "Gaussian 1737" in str(e.value), str(e.value)
+    good = gt.generate_raw(n=4000, seed=3, config="c1")
+    s, tg = _dev_scene(good, 0.05)
+    u = uniforms([0, 0.3, 4.0], [0, 0, 0], 64, 48, hfov_deg=50, alpha_min=0.05, k=8, T_min=0.001)
+    before = s.render(u, 64, 48)["rgba"]
+    with pytest.raises(gt.GTError) as e:
+        s.update_device(t.data_ptr())
+    assert e.value.code == -1 and "Gaussian 1737" in str(e.value), str(e.value)
+    s.update_device(tg.data_ptr())
+    np.testing.assert_array_equal(s.render(u, 64, 48)["rgba"], before)
+    h = gt.Scene(raw=good, alpha_min=0.05)
+    with pytest.raises(gt.GTError) as e:
+        h.refit_device(t.data_ptr())
+    assert e.value.code == -1 and "Gaussian 1737" in str(e.value), str(e.value)
+    h.refit_device(tg.data_ptr())
+    np.testing.assert_array_equal(h.render(u, 64, 48)["rgba"], before)

@@ -158,6 +158,33 @@ def test_invalid_records_rejected(label, raw):
     assert rc == -1 and "Gaussian 37" in gt.last_error() and not h
 
 
+def _overflow_raws():
+    """Finite records whose proxy is not finite in f32 (record 37)."""
+    base = gt.generate_raw(n=64, seed=3, config="c1")
+    base[37, 51] = 3.0  # opacity 0.95: the record has a proxy at alpha_min 0.05
+    cases = []
+    for label, col, val in [("position near FLT_MAX", 0, 3.0e38), ("scale x proxy factor overflows", 53, 88.5),
+                            ("inverse overflows", 52, -87.0)]:
+        r = base.copy()
+        r[37, col] = val
+        cases.append((label, r))
+    return cases
+
+
+@pytest.mark.parametrize("label,raw", _overflow_raws(), ids=[c[0] for c in _overflow_raws()])
+def test_proxy_overflow_rejected(label, raw):
+    """A record that gt_prep_valid accepts but whose proxy transform, inverse
+    or box (centre) overflows f32 is GT_E_INVALID naming it; the same record
+    below alpha_min (no proxy) is accepted."""
+    with pytest.raises(gt.GTError) as e:
+        gt.host_prepare(raw, 0.05)
+    assert e.value.code == -1 and "Gaussian 37" in str(e.value) and "proxy" in str(e.value)
+    r = raw.copy()
+    r[37, 51] = -10.0  # opacity 4.5e-5 < alpha_min: no proxy, nothing to overflow
+    h = gt.host_prepare(r, 0.05)
+    assert 37 not in h["proxy_ids"].tolist()
+
+
 def test_valid_extremes_accepted():
     """Finite records at the edges stay valid: a tiny but nonzero quaternion,
     log-scales whose exp is a normal f32, logits that saturate the sigmoid."""

@@ -31,7 +31,7 @@ CXX_FLAGS = [
 ]
 
 HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip", "gt_image.hip", "gt_noalloc.hip"]
-CXX_SOURCES = ["gt_scene.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp", "gt_png.cpp"]
+CXX_SOURCES = ["gt_scene.cpp", "gt_host_build.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp", "gt_png.cpp"]
 
 
 def _run(cmd, verbose):

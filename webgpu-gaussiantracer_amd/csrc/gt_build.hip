@@ -72,7 +72,7 @@
 
 /* counters (u32); CTL_LVL + L = fronts (4-wide nodes) of collapse level L */
 enum { CTL_NPROX = 0, CTL_CMIN = 1, CTL_CMAX = 4, CTL_NODES = 7, CTL_OVF = 9, CTL_ITEMS = 10, CTL_STALE = 11,
-       CTL_PNODES = 12, CTL_BAD = 13 /* first invalid Gaussian (gt_prep_valid), ~0 if none */, CTL_LVL = 16,
+       CTL_PNODES = 12, CTL_BAD = 13 /* first invalid Gaussian (gt_prep_valid, gt_prep_proxy_finite), ~0 if none */, CTL_LVL = 16,
        CTL_WORDS = CTL_LVL + GT_STACK_MAX + 4 };
 #define COLLAPSE_BATCH 8 /* collapse levels launched between host reads of the level counts */
 #define MID_LEAVES (16 * BUILD_BLOCK) /* leaves per refit phase-2 workgroup (LDS counters) */
@@ -231,6 +231,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_prep(const float* __restrict__ 
       }
       float lo[3], hi[3];
       gt_prep_box(m, lo, hi);
+      if (!gt_prep_proxy_finite(m, inv, lo, hi)) atomicMin(bad, i);
       B[0] = make_float4(lo[0], lo[1], lo[2], 1.0f);
       B[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
 #pragma unroll
@@ -1024,11 +1025,12 @@ static int read_ctl(gt_devbuild* w, hipStream_t st) {
   return GT_OK;
 }
 
-/* GT_E_INVALID naming the first Gaussian k_prep found invalid (gt_prep_valid) */
+/* GT_E_INVALID naming the first Gaussian k_prep found invalid (gt_prep_valid, gt_prep_proxy_finite) */
 static int check_bad(const gt_devbuild* w) {
   const uint32_t bad = w->host_ctl[CTL_BAD];
   if (bad == 0xffffffffu) return GT_OK;
-  gt_set_error("Gaussian %u: non-finite parameter, zero quaternion or a scale that is 0 or inf in f32", bad);
+  gt_set_error("Gaussian %u: non-finite parameter, zero quaternion, a scale that is 0 or inf in f32, or a proxy "
+               "transform or box that is not finite in f32", bad);
   return GT_E_INVALID;
 }
 

@@ -147,6 +147,22 @@ GT_HD void gt_prep_box(const float* m, float* lo_out, float* hi_out) {
   }
 }
 
+/* Whether a proxy can be built on: its transform, inverse, world box and
+ * box centre all finite in f32.  A record that gt_prep_valid accepts can
+ * still fail here: a position near the f32 limit whose padded box (or box
+ * centre) rounds to inf, a scale so large that scale x sqrt(2 ln(opacity /
+ * alpha_min)) overflows, or one so small that the inverse's determinant
+ * underflows to 0.  Such a Gaussian is an argument error, like those
+ * gt_prep_valid rejects. */
+GT_HD int gt_prep_proxy_finite(const float* o2w, const float* w2o, const float* lo, const float* hi) {
+  int ok = 1;
+  for (int j = 0; j < 12; ++j) ok &= (__builtin_isfinite(o2w[j]) && __builtin_isfinite(w2o[j])) ? 1 : 0;
+  for (int j = 0; j < 3; ++j)
+    ok &= (__builtin_isfinite(lo[j]) && __builtin_isfinite(hi[j]) && __builtin_isfinite(0.5f * (lo[j] + hi[j])))
+              ? 1 : 0;
+  return ok;
+}
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,19 +11,10 @@
 #include <mutex>
 #include <vector>
 
+#include "gt_host_build.h"
 #include "gt_internal.h"
 
 #define GT_WORK_SLOTS 32u
-
-struct HostScene {
-  uint32_t n = 0;
-  std::vector<float> raw;      // n x 59 (empty for packed input)
-  std::vector<double> act;     // n x 8: opacity, scale[3], quat[4] (JS doubles)
-  std::vector<float> packed;   // n x 80
-  // proxies, in original order
-  std::vector<uint32_t> proxy_ids;
-  std::vector<float> o2w, w2o;  // n' x 12
-};
 
 struct DevBuf {
   void* p = nullptr;
