@@ -30,6 +30,15 @@ CXX_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
 ]
 
+# Per-file hipcc flags.  gt_forward.hip: leave wave-uniform branches
+# unstructured.  The traversal loop's leaf/node branch is uniform (the node
+# reference is read once per wave); structurized, it became flow blocks whose
+# PHIs merged the register k-buffer with undefined values, and the register
+# allocator then copied the 32-register buffer into and out of every leaf
+# visit, and spilled 88 B per lane.  Unstructured: no copies, no spills,
+# 4 % less time per frame, the same results.
+FILE_FLAGS = {"gt_forward.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]}
+
 HIP_SOURCES = ["gt_forward.hip", "gt_backward.hip", "gt_build.hip", "gt_train.hip", "gt_image.hip", "gt_noalloc.hip"]
 CXX_SOURCES = ["gt_scene.cpp", "gt_host_build.cpp", "gt_util.cpp", "gt_batch.cpp", "gt_autotune.cpp", "gt_png.cpp"]
 
@@ -50,7 +59,7 @@ def build_lib(verbose=False):
     jobs = []
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s + ".o")
-        jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(s, []) + ["-c", os.path.join(CSRC, s), "-o", o])
     for s in CXX_SOURCES:
         o = os.path.join(BUILD, s + ".o")
         jobs.append(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
