@@ -285,9 +285,8 @@ template <int MAXK, int MODE>
 __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmin, float tmax, float slack_abs,
                              KBuf<MAXK>& kb, uint32_t& count, Counters& st, uint32_t* wst, const float* s_faces) {
   if (S.root == GT_NODE_EMPTY) return;
-  const uint32_t lane = __lane_id();
   const uint64_t live = __ballot(1);
-  const float lo_t = tmin * (1.0f - 2e-5f) - slack_abs;
+  const float lo_t = __builtin_canonicalizef(tmin * (1.0f - 2e-5f) - slack_abs);
   wst[0] = S.root;
   wst[1] = (uint32_t)live;
   wst[2] = (uint32_t)(live >> 32);
@@ -298,7 +297,8 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
     const uint32_t ref = __builtin_amdgcn_readfirstlane(wst[3 * sp]);
     const uint32_t mlo = __builtin_amdgcn_readfirstlane(wst[3 * sp + 1]);
     const uint32_t mhi = __builtin_amdgcn_readfirstlane(wst[3 * sp + 2]);
-    const bool in = ((lane < 32u ? mlo >> lane : mhi >> (lane - 32u)) & 1u) != 0u;
+    const uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+    const bool in = __builtin_amdgcn_inverse_ballot_w64(mask); /* this lane's bit of the entry's mask */
     if (ref & GT_LEAF_BIT) {
       const uint32_t first = ref & GT_LEAF_FIRST_MASK;
       const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
@@ -311,37 +311,48 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
       if (in) st.nodes++;
       typedef const __attribute__((address_space(4))) float cfloat;
       cfloat* N = as_const<cfloat>(S.nodes + ref);
-      const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
-      float rep[4];
+      const float hi_t = __builtin_canonicalizef(kth * (1.0f + 2e-5f) + slack_abs);
+      uint32_t key[4], cref[4];
       uint64_t hm[4];
-      uint32_t cref[4];
+      /* the lanes of the entry's mask whose slab interval for child c is
+       * not empty (ballot of the compare alone, masked on the scalar
+       * unit), and the push key: the entry distance of the first such
+       * lane as an unsigned key that orders like the float (wave-uniform,
+       * so the sort below runs on the scalar unit) */
+      auto child = [&](int c, float tnear, float tfar) {
+        const uint64_t filled = 0ull - (uint64_t)(cref[c] != GT_NODE_EMPTY); /* all ones or zero */
+        hm[c] = __builtin_amdgcn_ballot_w64(tnear <= tfar) & mask & filled;
+        const int fl = hm[c] ? __ffsll((unsigned long long)hm[c]) - 1 : 0;
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(tnear), fl);
+        key[c] = hm[c] ? (b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u)) : 0xffffffffu;
+      };
+      /* child c: row r of the node holds axis r of the four child boxes */
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cref[c] = __float_as_uint(N[24 + c]);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        /* child c: row r of the node holds axis r of the four child boxes */
-        cref[c] = __float_as_uint(N[24 + c]);
         const float ax = (N[c] - r.o.x) * r.inv.x, bx = (N[4 + c] - r.o.x) * r.inv.x;
         const float ay = (N[8 + c] - r.o.y) * r.inv.y, by = (N[12 + c] - r.o.y) * r.inv.y;
         const float az = (N[16 + c] - r.o.z) * r.inv.z, bz = (N[20 + c] - r.o.z) * r.inv.z;
-        const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t));
-        const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t));
-        const bool hit = in && tnear <= tfar && cref[c] != GT_NODE_EMPTY;
-        hm[c] = __ballot(hit);
-        const int fl = hm[c] ? __ffsll((unsigned long long)hm[c]) - 1 : 0;
-        rep[c] = hm[c] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tnear), fl)) : INFINITY;
+        child(c, fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), lo_t)),
+              fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), hi_t)));
       }
-      uint32_t ord[4] = {0u, 1u, 2u, 3u};
-#define GT_OSWAP(a, b)                                                     \
-    if (rep[ord[a]] < rep[ord[b]]) {                                         \
-      const uint32_t o_ = ord[a]; ord[a] = ord[b]; ord[b] = o_;              \
+      /* sort (key, child) descending, then push far to near */
+#define GT_OSWAP(a, b)                                                          \
+    {                                                                             \
+      const bool sw_ = key[a] < key[b];                                           \
+      const uint32_t k_ = key[a], r_ = cref[a];                                   \
+      const uint64_t m_ = hm[a];                                                  \
+      key[a] = sw_ ? key[b] : k_; cref[a] = sw_ ? cref[b] : r_; hm[a] = sw_ ? hm[b] : m_; \
+      key[b] = sw_ ? k_ : key[b]; cref[b] = sw_ ? r_ : cref[b]; hm[b] = sw_ ? m_ : hm[b]; \
     }
       GT_OSWAP(0, 1) GT_OSWAP(2, 3) GT_OSWAP(0, 2) GT_OSWAP(1, 3) GT_OSWAP(1, 2)
 #undef GT_OSWAP
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const uint32_t k = ord[c];
-        const uint64_t m = hm[k];
+        const uint64_t m = hm[c];
         if (m) {
-          wst[3 * sp] = cref[k];
+          wst[3 * sp] = cref[c];
           wst[3 * sp + 1] = (uint32_t)m;
           wst[3 * sp + 2] = (uint32_t)(m >> 32);
           ++sp;
