@@ -38,9 +38,24 @@ namespace {
 __constant__ float c_faces[12 * GT_NUM_FACES];
 
 #define GT_BLOCK 256
-#ifndef GT_MIN_WAVES
-#define GT_MIN_WAVES 4 /* waves per SIMD the register budget is sized for */
+/* Waves per SIMD the register budget of k_forward is sized for (512 /
+ * waves VGPRs per lane), by k-buffer size.  The packet kernel is bound by
+ * per-wave latency (stack pop -> scalar node load -> box tests -> sort ->
+ * push), so occupancy pays: at MAXK <= 16, five waves (96 VGPRs; the few
+ * spills sit outside the traversal loop) run the C2 frame 5 % faster than
+ * four (128 VGPRs, no spills), and six spill too much.  The 64 registers
+ * of a 32-slot buffer spill heavily at five waves, so MAXK 32 keeps four. */
+#ifndef GT_WAVES_K16
+#define GT_WAVES_K16 5
 #endif
+#ifndef GT_WAVES_K32
+#define GT_WAVES_K32 4
+#endif
+template <int MAXK>
+struct Waves {
+  static constexpr int value = MAXK <= 16 ? GT_WAVES_K16 : GT_WAVES_K32;
+};
+#define GT_RENDER_WAVES 4 /* k_render (per-lane traversal, tuning path 2) */
 #define GT_LDS_STACK 32 /* default LDS stack entries per lane; deeper ones spill to scratch */
 
 /* Traversal stack: the top GT_LDS_STACK entries live in LDS as a ring
@@ -584,7 +599,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
  * many cameras of uneven size keeps every CU busy to its end.  Each tile
  * is rendered by the same code either way. */
 template <int MAXK, int MODE, bool DUMP, bool STATS, bool BATCH, bool PACKET, bool PERSIST>
-__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
+__global__ __launch_bounds__(GT_BLOCK, Waves<MAXK>::value) void k_forward(const gt_dev_scene S, const gt_dev_frame F0,
                                                                     const gt_dev_frame* __restrict__ cams,
                                                                     uint32_t n_cams, uint32_t total_tiles,
                                                                     uint32_t tile_w, uint32_t lds_depth,
@@ -685,7 +700,7 @@ __device__ __forceinline__ void visit_node(const gt_dev_scene& S, uint32_t ref, 
 }
 
 template <int MAXK, bool BATCH, bool DUMP, bool STATS>
-__global__ __launch_bounds__(GT_BLOCK, GT_MIN_WAVES) void k_render(const gt_dev_scene S, const gt_dev_frame cam0,
+__global__ __launch_bounds__(GT_BLOCK, GT_RENDER_WAVES) void k_render(const gt_dev_scene S, const gt_dev_frame cam0,
                                                                    const gt_dev_frame* __restrict__ cams,
                                                                    uint32_t n_cams, uint32_t total_rays,
                                                                    unsigned int* work,
@@ -926,7 +941,7 @@ hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt
   dim3 grid((total_tiles - tile_offset + 3u) / 4u);
   if (PERSIST) {
     hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(GT_WORK_WORDS), 0, st, work);
-    const uint32_t g = (uint32_t)device_cus() * bpc;
+    const uint32_t g = (uint32_t)device_cus() * (bpc ? bpc : (uint32_t)Waves<MAXK>::value);
     grid = dim3(g < grid.x ? g : grid.x);
   }
   const dim3 block(GT_BLOCK);
@@ -1084,7 +1099,8 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
     gt_set_error("the persistent packet kernel (tuning path 4) needs its work counters");
     return GT_E_INVALID;
   }
-  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_MIN_WAVES;
+  /* 0: one resident block per wave slot (Waves<MAXK> per SIMD) */
+  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : 0u;
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
@@ -1137,7 +1153,7 @@ extern "C" int gt_launch_persistent(const gt_dev_scene* S, const gt_dev_frame* c
   const uint32_t rays = total_tiles * 64u;
   const uint32_t k = cam0->k;
   hipStream_t st = (hipStream_t)stream;
-  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_MIN_WAVES;
+  const uint32_t bpc = tune && tune->blocks_per_cu ? tune->blocks_per_cu : GT_RENDER_WAVES;
   const int grid = device_cus() * (int)bpc;
   const uint32_t refill = tune && tune->refill_min ? tune->refill_min : 48u;
   const uint32_t comp = tune && tune->comp_min ? tune->comp_min : 48u;
