@@ -302,17 +302,29 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
   if (S.root == GT_NODE_EMPTY) return;
   const uint64_t live = __ballot(1);
   const float lo_t = __builtin_canonicalizef(tmin * (1.0f - 2e-5f) - slack_abs);
-  wst[0] = S.root;
-  wst[1] = (uint32_t)live;
-  wst[2] = (uint32_t)(live >> 32);
-  int sp = 1;
+  /* The entry to visit next is held in scalar registers when it is the
+   * nearest child of the node just visited (which the LDS stack would have
+   * returned next anyway): no LDS store, load and read-back on a descent. */
+  uint32_t next_ref = S.root;
+  uint64_t next_mask = live;
+  bool have_next = true;
+  int sp = 0;
   float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1]; /* this lane's k-th distance */
-  while (sp > 0) {
-    --sp;
-    const uint32_t ref = __builtin_amdgcn_readfirstlane(wst[3 * sp]);
-    const uint32_t mlo = __builtin_amdgcn_readfirstlane(wst[3 * sp + 1]);
-    const uint32_t mhi = __builtin_amdgcn_readfirstlane(wst[3 * sp + 2]);
-    const uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+  for (;;) {
+    uint32_t ref;
+    uint64_t mask;
+    if (have_next) {
+      ref = next_ref;
+      mask = next_mask;
+      have_next = false;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      ref = __builtin_amdgcn_readfirstlane(wst[3 * sp]);
+      const uint32_t mlo = __builtin_amdgcn_readfirstlane(wst[3 * sp + 1]);
+      const uint32_t mhi = __builtin_amdgcn_readfirstlane(wst[3 * sp + 2]);
+      mask = ((uint64_t)mhi << 32) | mlo;
+    }
     const bool in = __builtin_amdgcn_inverse_ballot_w64(mask); /* this lane's bit of the entry's mask */
     if (ref & GT_LEAF_BIT) {
       const uint32_t first = ref & GT_LEAF_FIRST_MASK;
@@ -363,8 +375,11 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
     }
       GT_OSWAP(0, 1) GT_OSWAP(2, 3) GT_OSWAP(0, 2) GT_OSWAP(1, 3) GT_OSWAP(1, 2)
 #undef GT_OSWAP
+      /* misses (key ~0) sort first, so slot 3 holds the nearest hit child
+       * if any child was hit: it is visited next, the others go on the
+       * stack far to near */
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < 3; ++c) {
         const uint64_t m = hm[c];
         if (m) {
           wst[3 * sp] = cref[c];
@@ -372,6 +387,11 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
           wst[3 * sp + 2] = (uint32_t)(m >> 32);
           ++sp;
         }
+      }
+      if (hm[3]) {
+        next_ref = cref[3];
+        next_mask = hm[3];
+        have_next = true;
       }
     }
   }
