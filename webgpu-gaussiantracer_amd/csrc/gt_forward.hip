@@ -331,8 +331,19 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
       const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
       /* uniform loop, lane predicate inside: a divergent loop around the
        * k-buffer updates made the compiler copy (and spill) the buffer */
+      /* one dword of each of the leaf's 64-B records, all requested before
+       * the first record is needed, so that the record loads of the later
+       * proxies hit the scalar cache instead of waiting on L2 one after
+       * another (the value is consumed after the loop) */
+      typedef const __attribute__((address_space(4))) uint32_t cuint;
+      cuint* P = as_const<cuint>(S.prox) + 16 * (size_t)first + 12;
+      uint32_t touch[GT_LEAF_MAX];
+#pragma unroll
+      for (uint32_t q = 0; q < GT_LEAF_MAX; ++q) touch[q] = P[16 * (q < cnt ? q : cnt - 1u)];
       for (uint32_t q = 0; q < cnt; ++q)
         if (in) test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
+      asm volatile("" : : "s"(touch[0]), "s"(touch[1]), "s"(touch[2]), "s"(touch[3]), "s"(touch[4]),
+                   "s"(touch[5]), "s"(touch[6]), "s"(touch[7]));
       if (MODE != MODE_COUNT) kth = kb.t[MAXK - 1];
     } else {
       if (in) st.nodes++;
