@@ -293,12 +293,8 @@ GT_CONST_TABLE int8_t gt_ico_face_edges[60] = GT_ICO_FACE_EDGES;
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GT_UNROLL _Pragma("unroll")
-/* keeps the scheduler from evaluating later faces' edge values early (which
- * would keep all 30 live at once) */
-#define GT_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
 #else
 #define GT_UNROLL
-#define GT_SCHED_BARRIER()
 #endif
 
 /* The edge constants {a x b, b - a} of gt_ico_edge_tables and the face
@@ -388,41 +384,80 @@ GT_HD float gt_sparse_dot(gt_f3 a, float c0, float c1, float c2) {
 
 /* Face-major evaluation through vertex terms.  With w = o x d,
  *   F(a->b) = d . (a x b) + w . b - w . a,
- * so the 12 vertex terms W_v = w . v and the products of d with the few
+ * so the vertex terms W_v = w . v and the products of d with the few
  * distinct a x b components are formed once, and an edge value costs a few
- * adds.  A face (edges e1 e2 e3 with orientations) is a candidate iff
- * max(+-F_e1, +-F_e2, +-F_e3) <= margin_g.  The rounding error of this
- * evaluation is below 25 ulp * |d|_1 (|o|_1 + 1), inside margin_g with the
- * triangle test's own error.  A NaN edge value makes fmaxf ignore it (the
- * face stays a candidate unless all three are NaN, where the exact test
- * rejects too).  No facing pre-test: a back face passes only when all
- * three signed values lie in [0, margin_g] (its three F sum to
- * dot(n, d) > 0 with the opposite sign pattern), i.e. for faces nearly
- * parallel to the ray, and the exact test's facing rule rejects it;
- * margin_n is kept for the interface. */
+ * adds.  The icosahedron is centrally symmetric: each vertex v has the
+ * antipode -v (GT_ICO_VERTEX_BASE / GT_ICO_VERTEX_SIGN: v = sign * base
+ * vertex), so W_{-v} = -W_v exactly, and the edge (a, b) and its antipodal
+ * edge (-a, -b) share a x b.  For an antipodal pair (GT_ICO_EDGE_PAIRS:
+ * edge e, edge e', s)
+ *   F(e)  = d . K + D,   D = W_b - W_a,
+ *   F(e') = s (d . K - D),
+ * with s = -1 where the table stores e' as (-b, -a).  So 6 vertex terms,
+ * 15 dot products with a x b and 15 differences D give the 30 edge values.
+ * A face (edges e1 e2 e3 with orientations) is a candidate iff
+ * max(+-F_e1, +-F_e2, +-F_e3) - margin_g < 0; the sign bit of that value
+ * is shifted into the mask, from the last face to the first (face f at
+ * bit f).  The rounding error of this evaluation is below 25 ulp * |d|_1
+ * (|o|_1 + 1), inside margin_g with the triangle test's own error.  A NaN
+ * edge value makes fmaxf ignore it; a face whose three values are all NaN
+ * may come out either way, and the exact test rejects it.  No facing
+ * pre-test: a back face passes only when all three signed values lie in
+ * [0, margin_g] (its three F sum to dot(n, d) > 0 with the opposite sign
+ * pattern), i.e. for faces nearly parallel to the ray, and the exact
+ * test's facing rule rejects it; margin_n is kept for the interface. */
+#define GT_ICO_VERTEX_BASE {0, 1, 1, 0, 2, 3, 3, 2, 4, 5, 5, 4}
+#define GT_ICO_VERTEX_SIGN {1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1}
+#define GT_ICO_BASE_VERTS {0, 1, 4, 5, 8, 9}
+#define GT_ICO_EDGE_PAIRS                                                                                  \
+  {0, 28, 1, 1, 27, 1, 2, 26, 1, 3, 25, 1, 4, 24, -1, 5, 23, 1, 6, 22, 1, 7, 21, 1, 8, 20, 1, 9, 29, -1, \
+   10, 17, 1, 11, 14, 1, 12, 19, 1, 13, 16, -1, 15, 18, 1}
 GT_HD uint32_t gt_face_candidates(gt_f3 oo, gt_f3 od, float margin_g, float margin_n) {
   const float edges[6 * GT_NUM_EDGES] = GT_ICO_EDGE_VALUES;
   const uint8_t ev[2 * GT_NUM_EDGES] = GT_ICO_EDGE_VERTS;
   const float vtx[36] = GT_ICO_VERTEX_VALUES;
   const int8_t fe[3 * GT_NUM_FACES] = GT_ICO_FACE_EDGES;
+  const int8_t vbase[12] = GT_ICO_VERTEX_BASE, vsign[12] = GT_ICO_VERTEX_SIGN;
+  const uint8_t bverts[6] = GT_ICO_BASE_VERTS;
+  const int8_t pairs[45] = GT_ICO_EDGE_PAIRS;
   const gt_f3 w = gt_cross(oo, od);
-  float W[12];
+  float W6[6], F[GT_NUM_EDGES];
   (void)margin_n;
   GT_UNROLL
-  for (int v = 0; v < 12; ++v) W[v] = gt_sparse_dot(w, vtx[3 * v], vtx[3 * v + 1], vtx[3 * v + 2]);
+  for (int j = 0; j < 6; ++j) {
+    const int v = bverts[j];
+    W6[j] = gt_sparse_dot(w, vtx[3 * v], vtx[3 * v + 1], vtx[3 * v + 2]);
+  }
+  GT_UNROLL
+  for (int p = 0; p < 15; ++p) {
+    const int e = pairs[3 * p], e2 = pairs[3 * p + 1], sg = pairs[3 * p + 2];
+    const int a = ev[2 * e], b = ev[2 * e + 1];
+    const float* K = edges + 6 * e;
+    const float kd = gt_sparse_dot(od, K[0], K[1], K[2]);
+    const float wb = vsign[b] > 0 ? W6[vbase[b]] : -W6[vbase[b]];
+    const float wa = vsign[a] > 0 ? W6[vbase[a]] : -W6[vbase[a]];
+    const float D = wb - wa;
+    F[e] = kd + D;
+    const float f2 = kd - D;
+    F[e2] = sg > 0 ? f2 : -f2;
+  }
   uint32_t cand = 0;
   GT_UNROLL
-  for (int f = 0; f < GT_NUM_FACES; ++f) {
+  for (int f = GT_NUM_FACES - 1; f >= 0; --f) {
     float val[3];
     GT_UNROLL
     for (int j = 0; j < 3; ++j) {
       const int c = fe[3 * f + j], e = (c > 0 ? c : -c) - 1;
-      const float* K = edges + 6 * e;
-      const float F = (gt_sparse_dot(od, K[0], K[1], K[2]) + W[ev[2 * e + 1]]) - W[ev[2 * e]];
-      val[j] = c > 0 ? F : -F;
+      val[j] = c > 0 ? F[e] : -F[e];
     }
-    const float m = fmaxf(fmaxf(val[0], val[1]), val[2]);
-    cand |= (uint32_t)(m <= margin_g) << f;
+    const float m = fmaxf(fmaxf(val[0], val[1]), val[2]) - margin_g;
+#if defined(__HIP_DEVICE_COMPILE__)
+    cand = __builtin_amdgcn_alignbit(cand, __float_as_uint(m), 31u); /* (cand << 1) | sign bit of m */
+#else
+    uint32_t mb;
+    memcpy(&mb, &m, 4);
+    cand = (cand << 1) | (mb >> 31);
+#endif
   }
   return cand;
 }
@@ -463,6 +498,34 @@ static inline int gt_ico_literals_match(void) {
         const int c = lf[3 * f + j], e = (c > 0 ? c : -c) - 1;
         if (ef[2 * e + (c > 0 ? 0 : 1)] != f) return 0;
       }
+  }
+  {
+    /* antipodes: vertex v = sign * base vertex, and each edge pair maps onto
+     * itself under v -> -v with the stated orientation */
+    const int8_t vb[12] = GT_ICO_VERTEX_BASE, vs[12] = GT_ICO_VERTEX_SIGN;
+    const uint8_t bv[6] = GT_ICO_BASE_VERTS;
+    const int8_t pr[45] = GT_ICO_EDGE_PAIRS;
+    const uint8_t lv[2 * GT_NUM_EDGES] = GT_ICO_EDGE_VERTS;
+    int anti[12], seen[GT_NUM_EDGES] = {0};
+    for (int v = 0; v < 12; ++v) {
+      if (vb[v] < 0 || vb[v] > 5 || (vs[v] != 1 && vs[v] != -1)) return 0;
+      const gt_f3 P = gt_ico_vertex(v), B = gt_ico_vertex(bv[vb[v]]);
+      if (P.x != vs[v] * B.x || P.y != vs[v] * B.y || P.z != vs[v] * B.z) return 0;
+      anti[v] = -1;
+      for (int u = 0; u < 12; ++u) {
+        const gt_f3 Q = gt_ico_vertex(u);
+        if (Q.x == -P.x && Q.y == -P.y && Q.z == -P.z) anti[v] = u;
+      }
+      if (anti[v] < 0) return 0;
+    }
+    for (int p = 0; p < 15; ++p) {
+      const int e = pr[3 * p], e2 = pr[3 * p + 1], sg = pr[3 * p + 2];
+      if (e < 0 || e >= GT_NUM_EDGES || e2 < 0 || e2 >= GT_NUM_EDGES || seen[e] || seen[e2]) return 0;
+      seen[e] = seen[e2] = 1;
+      const int a = anti[lv[2 * e]], b = anti[lv[2 * e + 1]];
+      if (sg == 1 ? (lv[2 * e2] != a || lv[2 * e2 + 1] != b) : (sg != -1 || lv[2 * e2] != b || lv[2 * e2 + 1] != a))
+        return 0;
+    }
   }
   return 1;
 }

@@ -86,54 +86,50 @@ struct RayState {
   gt_f3 inv; /* slab-test reciprocals, tiny components clamped */
 };
 
-__device__ __forceinline__ bool key_lt(float ta, uint32_t ia, float tb, uint32_t ib) {
-  const uint32_t a = __float_as_uint(ta), b = __float_as_uint(tb);
-  return a < b || (a == b && ia < ib);
-}
-
 /* k-buffer in registers.  The k active slots are the LAST k of MAXK; the
  * first MAXK-k hold the key (+0, 0), smaller than every hit key (hits have
  * t > t_curr >= 0), so the insertion chain never moves them and the k-th
  * entry is always slot MAXK-1. */
 template <int MAXK>
 struct KBuf {
-  float t[MAXK];
-  uint32_t id[MAXK];
+  /* one 64-bit key per slot: t bits above, id below.  Hit distances are
+   * positive, so the unsigned order of the keys is the (t, id) order and a
+   * key compare is one 64-bit compare. */
+  uint64_t key[MAXK];
+  __device__ __forceinline__ float t_at(int i) const { return __uint_as_float((uint32_t)(key[i] >> 32)); }
+  __device__ __forceinline__ uint32_t id_at(int i) const { return (uint32_t)key[i]; }
+  __device__ __forceinline__ float kth() const { return t_at(MAXK - 1); }
 
   __device__ __forceinline__ void reset(uint32_t off, float t_max) {
-    /* opaque copies of the (uniform) inputs: the 2*MAXK reset values are
+    /* opaque copies of the (uniform) inputs: the MAXK reset values are
      * recomputed in each round instead of hoisted out of the round loop and
      * kept live (spilled) across the traversal */
     asm volatile("" : "+v"(off), "+v"(t_max));
+    const uint64_t sentinel = ((uint64_t)__float_as_uint(t_max) << 32) | GT_SENTINEL;
 #pragma unroll
-    for (int i = 0; i < MAXK; ++i) {
-      const bool dummy = (uint32_t)i < off;
-      t[i] = dummy ? 0.0f : t_max;
-      id[i] = dummy ? 0u : GT_SENTINEL;
-    }
+    for (int i = 0; i < MAXK; ++i) key[i] = (uint32_t)i < off ? 0ull : sentinel;
   }
-  /* GS_shader.ts:349-353 with the key (t, id): stable for equal keys */
+  /* GS_shader.ts:349-353 with the key (t, id): stable for equal keys.  The
+   * buffer is sorted, so the reference's swap chain shifts every entry from
+   * the insertion point on by one slot: slot i takes slot i-1 if h precedes
+   * both, h if it precedes slot i only.  (Where the chain stops at two equal
+   * keys, the entries are identical, so shifting them changes nothing.)
+   * Written from the last slot down, every slot is updated in place, with
+   * no copies of the buffer. */
   __device__ __forceinline__ void insert(float ht, uint32_t hid) {
-    if (!key_lt(ht, hid, t[MAXK - 1], id[MAXK - 1])) return;
+    const uint64_t h = ((uint64_t)__float_as_uint(ht) << 32) | hid;
+    if (!(h < key[MAXK - 1])) return;
 #pragma unroll
-    for (int i = 0; i < MAXK; ++i) {
-      const bool sw = key_lt(ht, hid, t[i], id[i]);
-      const float tt = t[i];
-      const uint32_t ti = id[i];
-      t[i] = sw ? ht : tt;
-      id[i] = sw ? hid : ti;
-      ht = sw ? tt : ht;
-      hid = sw ? ti : hid;
+    for (int i = MAXK - 1; i > 0; --i) {
+      const bool lt = h < key[i], lp = h < key[i - 1];
+      key[i] = lt ? (lp ? key[i - 1] : h) : key[i];
     }
+    key[0] = h < key[0] ? h : key[0];
   }
   __device__ __forceinline__ void shift_left(int s) {
 #pragma unroll
-    for (int i = 0; i + 1 < MAXK; ++i) {
-      if (i + s < MAXK) {
-        t[i] = t[i + s];
-        id[i] = id[i + s];
-      }
-    }
+    for (int i = 0; i + 1 < MAXK; ++i)
+      if (i + s < MAXK) key[i] = key[i + s];
   }
 };
 
@@ -172,6 +168,18 @@ __device__ __forceinline__ void load_proxy(P4* P, float* w, uint32_t* gid) {
   *gid = __float_as_uint(q3.x);
 }
 
+/* gt_xform_dir without its "+ c3 * 0" terms.  For a finite c3 the term is
+ * +0 or -0, and x + (+-0) equals x except that x = -0 may become +0.  A
+ * component's sign of zero changes no outcome of the proxy test: the
+ * triangle test's t does not depend on the direction, dot(n, d) = +-0
+ * rejects with either sign (t becomes +-inf or NaN), and beta, gamma and
+ * the conservative filters compare sums against 0 and 1, where -0 and +0
+ * compare equal.  So the hits are those of gt_xform_dir. */
+__device__ __forceinline__ gt_f3 xform_dir_nz(const float* m, gt_f3 d) {
+  return gt_f3_make((m[0] * d.x + m[3] * d.y) + m[6] * d.z, (m[1] * d.x + m[4] * d.y) + m[7] * d.z,
+                    (m[2] * d.x + m[5] * d.y) + m[8] * d.z);
+}
+
 template <int MAXK, int MODE, bool UNIFORM = false>
 __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, const RayState& r, float tmin,
                                            float tmax, KBuf<MAXK>& kb, uint32_t& count, Counters& st,
@@ -184,7 +192,7 @@ __device__ __forceinline__ void test_proxy(const gt_dev_scene& S, uint32_t pi, c
     load_proxy<float4>(reinterpret_cast<const float4*>(S.prox) + 4 * (size_t)pi, w, &gid);
   st.proxies++;
   const gt_f3 oo = gt_xform_point(w, r.o);
-  const gt_f3 od = gt_xform_dir(w, r.d);
+  const gt_f3 od = xform_dir_nz(w, r.d);
   /* conservative bounding-sphere test: every face lies inside the
    * circumsphere; the margin covers the FP32 error of the cross product */
   {
@@ -247,7 +255,7 @@ __device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, floa
       const float4* N = reinterpret_cast<const float4*>(S.nodes + ref);
       const float4 lx = N[0], hx = N[1], ly = N[2], hy = N[3], lz = N[4], hz = N[5];
       const uint4 ch = reinterpret_cast<const uint4*>(N)[6];
-      const float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1];
+      const float kth = (MODE == MODE_COUNT) ? tmax : kb.kth();
       const float hi_t = kth * (1.0f + 2e-5f) + slack_abs;
       const float clx[4] = {lx.x, lx.y, lx.z, lx.w}, chx[4] = {hx.x, hx.y, hx.z, hx.w};
       const float cly[4] = {ly.x, ly.y, ly.z, ly.w}, chy[4] = {hy.x, hy.y, hy.z, hy.w};
@@ -309,7 +317,7 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
   uint64_t next_mask = live;
   bool have_next = true;
   int sp = 0;
-  float kth = (MODE == MODE_COUNT) ? tmax : kb.t[MAXK - 1]; /* this lane's k-th distance */
+  float kth = (MODE == MODE_COUNT) ? tmax : kb.kth(); /* this lane's k-th distance */
   for (;;) {
     uint32_t ref;
     uint64_t mask;
@@ -344,7 +352,7 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
         if (in) test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
       asm volatile("" : : "s"(touch[0]), "s"(touch[1]), "s"(touch[2]), "s"(touch[3]), "s"(touch[4]),
                    "s"(touch[5]), "s"(touch[6]), "s"(touch[7]));
-      if (MODE != MODE_COUNT) kth = kb.t[MAXK - 1];
+      if (MODE != MODE_COUNT) kth = kb.kth();
     } else {
       if (in) st.nodes++;
       typedef const __attribute__((address_space(4))) float cfloat;
@@ -521,7 +529,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
         kb.reset(off, F.t_max);
         if (PACKET) trace_packet<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
         else trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
-        t_curr = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
+        t_curr = kb.kth() + GT_RELAUNCH_EPS;
         /* bring the active slots to the front (uniform shifts) */
         if (off & 1u) kb.shift_left(1);
         if (off & 2u) kb.shift_left(2);
@@ -534,7 +542,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
          * per entry (static register moves), so the loop need not unroll */
 #pragma unroll 1
         for (uint32_t j = 0; j < k; ++j) {
-          const uint32_t id = kb.id[0];
+          const uint32_t id = kb.id_at(0);
           if (DUMP && rounds < MR) F.hit_ids[((size_t)ray * MR + rounds) * k + j] = id;
           if (!stop) {
             if (id == GT_SENTINEL) {
@@ -574,7 +582,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
       kb.reset(0, F.t_max);
       if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
       else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
-      const uint32_t id = kb.id[MAXK - 1];
+      const uint32_t id = kb.id_at(MAXK - 1);
       if (id != GT_SENTINEL) L = radiance(S.sh + 48 * (size_t)id, F.sh_degree, r.d);
       rounds = 1;
       if (DUMP && MR > 0) {
@@ -855,7 +863,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_RENDER_WAVES) void k_render(const gt_d
         const size_t MR = DUMP ? CAM(max_rounds) : 0;
 #pragma unroll 1
         for (uint32_t jj = 0; jj < k; ++jj) {
-          const uint32_t id = kb.id[0];
+          const uint32_t id = kb.id_at(0);
           if (DUMP && rounds < MR) CAM(hit_ids)[((size_t)pix * MR + rounds) * k + jj] = id;
           if (!stop) {
             if (id == GT_SENTINEL) {
@@ -903,7 +911,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_RENDER_WAVES) void k_render(const gt_d
           break;
         }
         st.nodes++;
-        visit_node<MAXK>(S, ref, r, lo_t, kb.t[MAXK - 1] * (1.0f + 2e-5f) + slack, stk);
+        visit_node<MAXK>(S, ref, r, lo_t, kb.kth() * (1.0f + 2e-5f) + slack, stk);
       }
       if (pcount > 0) {
         test_proxy<MAXK, MODE_RENDER>(S, pfirst, r, t_curr, t_max, kb, dummy_count, st, s_faces);
@@ -911,7 +919,7 @@ __global__ __launch_bounds__(GT_BLOCK, GT_RENDER_WAVES) void k_render(const gt_d
         --pcount;
       } else {
         /* round traversal done: t_curr = t_k + 1e-4 (GS_shader.ts:248) */
-        t_next = kb.t[MAXK - 1] + GT_RELAUNCH_EPS;
+        t_next = kb.kth() + GT_RELAUNCH_EPS;
         if (off & 1u) kb.shift_left(1);
         if (off & 2u) kb.shift_left(2);
         if (off & 4u) kb.shift_left(4);
