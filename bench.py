@@ -42,7 +42,6 @@ BVH variant) on every core the process may use, on a seeded 1/3 sample of
 the rays.
 """
 import argparse
-import argparse
 import json
 import os
 import sys
@@ -111,6 +110,17 @@ def measure_traffic(args, timeout_s=120):
                 if r.returncode != 0:
                     return None, f"rocprofv3 --pmc {counter} ({name}) exited {r.returncode}: {r.stderr[-300:]}"
                 out[(name, counter)] = _per_dispatch(_counter_csv_rows(d), counter)
+        # VALU issue of the same launches: instructions, active quad-cycles,
+        # thread-cycles and the busy cycles of the XCDs (one pass; SQ and
+        # GRBM blocks within their counter limits)
+        d = os.path.join(tmp, "probe_valu")
+        r = subprocess.run([prof, "--pmc"] + list(VALU_COUNTERS) + ["-d", d, "-o", "run", "-f", "csv", "--"] + probe,
+                           capture_output=True, text=True, timeout=timeout_s, env=env)
+        if r.returncode != 0:
+            return None, f"rocprofv3 --pmc VALU counters exited {r.returncode}: {r.stderr[-300:]}"
+        rows = _counter_csv_rows(d)
+        out["valu"] = {c: _per_dispatch(rows, c) for c in VALU_COUNTERS}
+        out["agent"] = _agent_info(d)
         d = os.path.join(tmp, "probe_trace")
         r = subprocess.run([prof, "--kernel-trace", "-d", d, "-o", "run", "-f", "csv", "--"] + probe,
                            capture_output=True, text=True, timeout=timeout_s, env=env)
@@ -142,7 +152,49 @@ def measure_traffic(args, timeout_s=120):
             bwd[kname] = int(sum(v) * 1024.0 * scale)
         res[counter]["backward_bytes"] = bwd
     res["kernel_ms"] = out["trace"]
+    res["valu"] = _valu_issue(out["valu"], out["agent"], out["trace"].get("k_forward", [])[:2])
     return res, None
+
+
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE")
+
+
+def _agent_info(outdir):
+    """(SIMDs, XCDs) of the GPU agent from rocprofv3's agent_info.csv."""
+    import csv
+    import glob
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*agent_info.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if int(r.get("Simd_Count") or 0) > 0:
+                    return int(r["Simd_Count"]), int(r.get("Num_Xcc") or 1)
+    return None
+
+
+def _valu_issue(valu, agent, trace_ms):
+    """VALU issue of the forward kernel (mean of the first two plain launches):
+    instructions, the fraction of the launch every SIMD's VALU is busy
+    (SQ_ACTIVE_INST_VALU counts quad-cycles: a wave64 VALU instruction
+    occupies a SIMD for 4 cycles), lane efficiency and the clock over the
+    launch (GRBM_GUI_ACTIVE summed over the XCDs, per XCD, over the traced
+    duration of the same launches)."""
+    if not agent:
+        return None
+    simds, xcds = agent
+    v = {}
+    for c in VALU_COUNTERS:
+        fwd = [x for n, x in valu[c] if "k_forward<" in n][:2]
+        if len(fwd) < 2:
+            return None
+        v[c] = sum(fwd) / len(fwd)
+    cycles = v["GRBM_GUI_ACTIVE"] / xcds
+    out = {"valu_insts": int(v["SQ_INSTS_VALU"]),
+           "busy_frac": round(4.0 * v["SQ_ACTIVE_INST_VALU"] / (simds * cycles), 4) if cycles else None,
+           "lane_efficiency": round(v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_ACTIVE_INST_VALU"]), 4),
+           "cycles_per_xcd": int(cycles), "simds": simds, "xcds": xcds}
+    if len(trace_ms) >= 2:
+        out["clock_ghz"] = round(cycles / (sum(trace_ms) / len(trace_ms) * 1e-3) / 1e9, 3)
+    return out
 
 
 def _kernel_trace_ms(outdir):
@@ -808,6 +860,13 @@ def main():
                                 "measure_s": round(t_tr, 1)} if traffic else {"reason": traffic_why}),
             "algorithmic_bytes": int(B_fwd), "kernel": FWD_KERNEL,
             "measured_copy_gbs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5),
+            # the bound the kernel meets first: VALU issue (no dense contraction,
+            # so no MFMA; HBM traffic is a small fraction of its peak)
+            "valu_issue": (dict(traffic["valu"], method="rocprofv3 --pmc " + " ".join(VALU_COUNTERS) +
+                                " child run of this script (--traffic-probe: mean of the first two plain "
+                                "launches); busy_frac = 4 SQ_ACTIVE_INST_VALU / (SIMDs x GRBM_GUI_ACTIVE "
+                                "per XCD)")
+                           if traffic and traffic.get("valu") else None),
         },
         "stats": st,
         "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
