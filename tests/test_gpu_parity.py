@@ -44,6 +44,37 @@ def test_cluster_scene_camera_inside():
     assert g["stats"]["rounds"] > g["stats"]["rays"]  # multi-round rays exist
 
 
+@pytest.mark.parametrize("k,gpu_build", [(1, False), (3, False), (4, True), (16, False), (16, True)])
+def test_exact_t_ties_ordered_by_id(k, gpu_build):
+    """Every Gaussian three times, under shuffled ids: the copies have
+    identical proxies, so each hit comes with two exact-t ties.  The
+    k-buffer orders them by Gaussian id (the reference by its traversal
+    order, which only a run of the reference could pin); ties fall inside
+    rounds and across the k-th slot, where the relaunch at t_k + 1e-4 skips
+    the remaining copies.  Hit lists bit-exact against the oracle (which
+    orders by the same key) on the host-built and the GPU-built tree (the
+    radix tree sees equal Morton codes)."""
+    _print(f"case exact-t ties k={k} gpu_build={gpu_build}")
+    base = gt.generate_raw(n=1500, seed=9, config="c1")
+    n = base.shape[0]
+    perm = np.random.default_rng(9).permutation(3 * n)
+    raw = np.concatenate([base, base, base])[perm]
+    orig = perm % n  # Gaussian id -> index of the copied Gaussian
+    s = gt.Scene(raw, alpha_min=0.1, gpu_build=gpu_build)
+    _, o = scene_pair(raw, 0.1, gpu=False)
+    u = uniforms([0.2, -0.1, 8], [0, 0, 0], 64, 48, hfov_deg=50, alpha_min=0.1, k=k, T_min=0.01)
+    g, _ = check_parity(s, o, u, 64, 48, max_rounds=256, label=f"ties k{k}")
+    # the ties occur, and in id order
+    h = g["hit_ids"].reshape(-1, k)
+    a, b = h[:, :-1], h[:, 1:]
+    valid = (a != gt.SENTINEL) & (b != gt.SENTINEL)
+    tie = valid & (orig[np.where(valid, a, 0)] == orig[np.where(valid, b, 0)])
+    if k > 1:
+        assert tie.sum() > 100, f"only {tie.sum()} adjacent tied entries"
+        assert np.all(a[tie] < b[tie])
+    s.close()
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 13, 16, 24, 32])
 def test_k_sweep(k):
     _print(f"case k sweep k={k}")
