@@ -236,7 +236,7 @@ GT_HD gt_face gt_ico_face(int f) {
  * 1 - beta - gamma = F(p1->p2) / dot(n, d) (the three F sum to dot(n, d)),
  * so a FRONT face (dot(n, d) < 0) can pass only if all three F <= 0.
  * gt_face_candidates evaluates the 30 edge values once, without division,
- * and returns the mask of faces whose three signed values are <= margin_g.
+ * and returns the mask of faces whose three signed values are below margin_g.
  * With margin_g far above the FP32 error of the triangle test, every face
  * the exact test + facing rule accepts is in the mask; the exact test then
  * runs only on those faces. */

@@ -88,7 +88,7 @@ struct RayState {
 
 /* k-buffer in registers.  The k active slots are the LAST k of MAXK; the
  * first MAXK-k hold the key (+0, 0), smaller than every hit key (hits have
- * t > t_curr >= 0), so the insertion chain never moves them and the k-th
+ * t > t_curr >= 0), so the insertion never moves them and the k-th
  * entry is always slot MAXK-1. */
 template <int MAXK>
 struct KBuf {
