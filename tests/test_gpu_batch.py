@@ -253,3 +253,29 @@ def test_one_scene_on_two_streams():
         for rep in range(6):
             assert imgs[i][rep].cpu().numpy().tobytes() == ref[i].tobytes(), f"stream {i} render {rep}"
     s.close()
+
+
+def test_autotune_frame_keeps_results_and_stores_its_choice():
+    """gt_autotune_frame times the launch candidates on the caller's own
+    frame and stores the fastest in the scene: the frame rendered after it
+    (with the stored tuning) equals the frame rendered before it bit for bit,
+    the scene reports the chosen tuning, and the times it returns are
+    those of real launches."""
+    print("case autotune_frame", flush=True)
+    s, c = _scene()
+    u = _cams(c, k=16)[2]
+    w, h = SIZES[2]
+    before = s.render(u, w, h, max_rounds=32)
+    dev = torch.device("cuda:0")
+    img = torch.zeros((h, w, 4), device=dev)
+    res = gt.autotune_frame(s, u, w, h, img.data_ptr(), iters=2,
+                            stream_ptr=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert res["n_tried"] >= 5 and 0.0 < res["best_ms"] <= res["first_ms"]
+    assert s.get_tuning() == res["best"]
+    assert img.cpu().numpy().tobytes() == before["rgba"].tobytes()  # its frames are the caller's frame
+    after = s.render(u, w, h, max_rounds=32)
+    for key in ("hit_ids", "accept", "n_rounds"):
+        assert np.array_equal(after[key], before[key]), key
+    assert after["rgba"].tobytes() == before["rgba"].tobytes()
+    s.close()
