@@ -90,6 +90,27 @@ struct RayState {
  * first MAXK-k hold the key (+0, 0), smaller than every hit key (hits have
  * t > t_curr >= 0), so the insertion never moves them and the k-th
  * entry is always slot MAXK-1. */
+/* Max and min of two k-buffer keys with the FP64 max/min instructions.  A
+ * key (t bits above, id below; t >= +0 and not NaN: hit distances, +0 in
+ * the dummy slots, t_max, which may be +inf, in the sentinels) read as a
+ * double has sign 0 and an exponent field that is the float's exponent and
+ * top three mantissa bits, all ones only for a float NaN: a non-negative,
+ * finite double.  Such doubles order like their bit patterns, including
+ * denormals (the kernels keep FP64 denormals), so v_max_f64 / v_min_f64
+ * return the larger / smaller key, bit for bit (they return one of their
+ * operands).  Inline ALU asm keeps the compiler from adding NaN
+ * canonicalisations. */
+__device__ __forceinline__ uint64_t key_max(uint64_t a, uint64_t b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(__longlong_as_double((long long)a)), "v"(__longlong_as_double((long long)b)));
+  return (uint64_t)__double_as_longlong(r);
+}
+__device__ __forceinline__ uint64_t key_min(uint64_t a, uint64_t b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(__longlong_as_double((long long)a)), "v"(__longlong_as_double((long long)b)));
+  return (uint64_t)__double_as_longlong(r);
+}
+
 template <int MAXK>
 struct KBuf {
   /* one 64-bit key per slot: t bits above, id below.  Hit distances are
@@ -111,20 +132,20 @@ struct KBuf {
   }
   /* GS_shader.ts:349-353 with the key (t, id): stable for equal keys.  The
    * buffer is sorted, so the reference's swap chain shifts every entry from
-   * the insertion point on by one slot: slot i takes slot i-1 if h precedes
-   * both, h if it precedes slot i only.  (Where the chain stops at two equal
-   * keys, the entries are identical, so shifting them changes nothing.)
-   * Written from the last slot down, every slot is updated in place, with
-   * no copies of the buffer. */
+   * the insertion point on by one slot: slot i becomes slot i-1 if h
+   * precedes both, h if it precedes slot i only, else stays, which is
+   *   slot'[i] = min(slot[i], max(slot[i-1], h)),  slot'[0] = min(slot[0], h).
+   * (Where the chain stops at two equal keys the entries are identical, so
+   * shifting them changes nothing.)  Written from the last slot down, every
+   * slot is updated in place: one FP64 max and one min per slot, with no
+   * lane masks (a compare-and-select form needs two compares, four
+   * selects and two wait states on the compare's mask per slot). */
   __device__ __forceinline__ void insert(float ht, uint32_t hid) {
     const uint64_t h = ((uint64_t)__float_as_uint(ht) << 32) | hid;
     if (!(h < key[MAXK - 1])) return;
 #pragma unroll
-    for (int i = MAXK - 1; i > 0; --i) {
-      const bool lt = h < key[i], lp = h < key[i - 1];
-      key[i] = lt ? (lp ? key[i - 1] : h) : key[i];
-    }
-    key[0] = h < key[0] ? h : key[0];
+    for (int i = MAXK - 1; i > 0; --i) key[i] = key_min(key[i], key_max(key[i - 1], h));
+    key[0] = key_min(key[0], h);
   }
   __device__ __forceinline__ void shift_left(int s) {
 #pragma unroll
