@@ -29,6 +29,7 @@
 #include "../../include/gt.h"
 #include "../../include/gt_fp32.h"
 #include "gt_internal.h"
+#include "gt_rect.h"
 
 namespace {
 
@@ -325,9 +326,17 @@ __device__ void trace(const gt_dev_scene& S, const RayState& r, float tmin, floa
  * those hits lie beyond the k-th key and the insertion rejects them, so
  * every lane ends with exactly the hits of trace().  Children are pushed
  * far to near by the entry distance of the first lane that hits them. */
+/* A wave's tile as a pixel rectangle, and the frame's proxy rectangles
+ * (gt_rect.h; NULL: no screen-space culling) */
+struct TileRect {
+  const uint32_t* rects;
+  uint32_t lo, hi; /* the tile packed like a rectangle (gt_rect.h) */
+};
+
 template <int MAXK, int MODE>
 __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmin, float tmax, float slack_abs,
-                             KBuf<MAXK>& kb, uint32_t& count, Counters& st, uint32_t* wst, const float* s_faces) {
+                             KBuf<MAXK>& kb, uint32_t& count, Counters& st, uint32_t* wst, const float* s_faces,
+                             const TileRect& tr) {
   if (S.root == GT_NODE_EMPTY) return;
   const uint64_t live = __ballot(1);
   const float lo_t = __builtin_canonicalizef(tmin * (1.0f - 2e-5f) - slack_abs);
@@ -369,8 +378,42 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
       uint32_t touch[GT_LEAF_MAX];
 #pragma unroll
       for (uint32_t q = 0; q < GT_LEAF_MAX; ++q) touch[q] = P[16 * (q < cnt ? q : cnt - 1u)];
-      for (uint32_t q = 0; q < cnt; ++q)
-        if (in) test_proxy<MAXK, MODE, true>(S, first + q, r, tmin, tmax, kb, count, st, s_faces);
+      /* The leaf's proxies whose screen rectangle meets the tile; the others
+       * fail the bounding-sphere stage in every lane.  The leaf's eight
+       * entries come in one 64-B scalar load (the table holds GT_LEAF_MAX
+       * entries past the last proxy, and scalar loads need 4-B alignment
+       * only); the empty asm holds them in registers ahead of the
+       * count-dependent tests, so the load is not split into the branches
+       * and waited on once. */
+      uint32_t todo = (2u << (cnt - 1u)) - 1u;
+      if (tr.rects) {
+        typedef uint32_t v16u __attribute__((ext_vector_type(16)));
+        typedef const __attribute__((address_space(4))) v16u __attribute__((aligned(4))) cv16u;
+        const v16u W16 = *as_const<cv16u>(tr.rects + 2 * (size_t)first);
+        uint32_t w[2 * GT_LEAF_MAX];
+#pragma unroll
+        for (uint32_t q = 0; q < 2 * GT_LEAF_MAX; ++q) w[q] = W16[q];
+        asm volatile("" : "+s"(w[0]), "+s"(w[1]), "+s"(w[2]), "+s"(w[3]), "+s"(w[4]), "+s"(w[5]), "+s"(w[6]),
+                     "+s"(w[7]), "+s"(w[8]), "+s"(w[9]), "+s"(w[10]), "+s"(w[11]), "+s"(w[12]), "+s"(w[13]),
+                     "+s"(w[14]), "+s"(w[15]));
+        uint32_t m = 0;
+#define GT_RQ(q) m |= gt_rect_meets(tr.lo, tr.hi, w[2 * q], w[2 * q + 1]) << q
+        switch (cnt) { /* the leaf's own entries only (3.2 per leaf at 1M) */
+          case 8: GT_RQ(7); [[fallthrough]];
+          case 7: GT_RQ(6); [[fallthrough]];
+          case 6: GT_RQ(5); [[fallthrough]];
+          case 5: GT_RQ(4); [[fallthrough]];
+          case 4: GT_RQ(3); [[fallthrough]];
+          case 3: GT_RQ(2); [[fallthrough]];
+          case 2: GT_RQ(1); [[fallthrough]];
+          default: GT_RQ(0);
+        }
+#undef GT_RQ
+        todo &= m;
+      }
+      for (; todo; todo &= todo - 1u)
+        if (in) test_proxy<MAXK, MODE, true>(S, first + (uint32_t)__builtin_ctz(todo), r, tmin, tmax, kb, count, st,
+                                             s_faces);
       asm volatile("" : : "s"(touch[0]), "s"(touch[1]), "s"(touch[2]), "s"(touch[3]), "s"(touch[4]),
                    "s"(touch[5]), "s"(touch[6]), "s"(touch[7]));
       if (MODE != MODE_COUNT) kth = kb.kth();
@@ -508,7 +551,8 @@ template <int MAXK, int MODE, bool DUMP, bool BATCH, bool PACKET>
 __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_frame& F0,
                                             const gt_dev_frame* __restrict__ cams, uint32_t n_cams,
                                             uint32_t total_tiles, uint32_t tile_w, uint32_t tile, uint32_t lane,
-                                            Stack& stk, uint32_t* wst, const float* s_faces, Counters& st) {
+                                            Stack& stk, uint32_t* wst, const float* s_faces, Counters& st,
+                                            const uint32_t* rects) {
   const uint32_t cam = (BATCH && tile < total_tiles) ? find_camera(cams, n_cams, tile) : 0u;
   const gt_dev_frame& F = BATCH ? cams[cam] : F0;
   const uint32_t tile_h = 64u / tile_w;
@@ -516,6 +560,13 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
   const uint32_t px = (lt % F.tiles_x) * tile_w + (lane % tile_w);
   const uint32_t py = (lt / F.tiles_x) * tile_h + (lane / tile_w);
   const bool active = tile < total_tiles && px < F.width && py < F.height;
+  TileRect tr;
+  tr.rects = rects;
+  {
+    const uint32_t tx0 = (lt % F.tiles_x) * tile_w, ty0 = (lt / F.tiles_x) * tile_h;
+    tr.lo = tx0 | ty0 << 16;
+    tr.hi = ((tx0 + tile_w - 1u) | (ty0 + tile_h - 1u) << 16) | GT_RECT_GUARD;
+  }
   if (active) {
     const uint32_t ray = py * F.width + px;
     RayState r;
@@ -548,7 +599,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
       while (F.T_min < T && t_curr < F.t_max) {
         if (F.early_stop == step++) break;
         kb.reset(off, F.t_max);
-        if (PACKET) trace_packet<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+        if (PACKET) trace_packet<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
         else trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
         t_curr = kb.kth() + GT_RELAUNCH_EPS;
         /* bring the active slots to the front (uniform shifts) */
@@ -601,7 +652,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
     } else if (MODE == MODE_CLOSEST) {
       /* debugMode 1 (GS_shader.ts:289-296, 312-321) */
       kb.reset(0, F.t_max);
-      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
       else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t id = kb.id_at(MAXK - 1);
       if (id != GT_SENTINEL) L = radiance(S.sh + 48 * (size_t)id, F.sh_degree, r.d);
@@ -613,7 +664,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
     } else {
       /* debugMode 2 (GS_shader.ts:201-228): front-face hits in (t_min, t_max) */
       kb.reset(0, F.t_max);
-      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces);
+      if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
       else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t c = count;
       float cr, cg, cb;
@@ -663,7 +714,8 @@ __global__ __launch_bounds__(GT_BLOCK, Waves<MAXK>::value) void k_forward(const 
                                                                     const gt_dev_frame* __restrict__ cams,
                                                                     uint32_t n_cams, uint32_t total_tiles,
                                                                     uint32_t tile_w, uint32_t lds_depth,
-                                                                    uint32_t tile_offset, unsigned int* work) {
+                                                                    uint32_t tile_offset, unsigned int* work,
+                                                                    const uint32_t* rects) {
   extern __shared__ uint32_t s_stack[];
   __shared__ float s_faces[12 * GT_NUM_FACES];
   for (uint32_t i = threadIdx.x; i < 12 * GT_NUM_FACES; i += GT_BLOCK) s_faces[i] = c_faces[i];
@@ -677,9 +729,11 @@ __global__ __launch_bounds__(GT_BLOCK, Waves<MAXK>::value) void k_forward(const 
   Counters st;
   if (!PERSIST) {
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t tile = tile_offset + blk * 4u + (threadIdx.x >> 6);
+    /* the same in every lane of the wave; readfirstlane tells the compiler,
+     * so that the tile's values live in scalar registers */
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(tile_offset + blk * 4u + (threadIdx.x >> 6));
     render_tile<MAXK, MODE, DUMP, BATCH, PACKET>(S, F0, cams, n_cams, total_tiles, tile_w, tile, lane, stk, wst,
-                                                 s_faces, st);
+                                                 s_faces, st, rects);
   } else {
     const uint32_t n = total_tiles - tile_offset;
     const uint32_t x0 = blockIdx.x & (GT_XCDS - 1u);
@@ -693,7 +747,7 @@ __global__ __launch_bounds__(GT_BLOCK, Waves<MAXK>::value) void k_forward(const 
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= hi - lo) break; /* every wave of the grid reaches this exit once the ranges are claimed */
         render_tile<MAXK, MODE, DUMP, BATCH, PACKET>(S, F0, cams, n_cams, total_tiles, tile_w, lo + t, lane, stk,
-                                                     wst, s_faces, st);
+                                                     wst, s_faces, st, rects);
       }
     }
   }
@@ -997,7 +1051,8 @@ int device_cus();
 template <int MAXK, int MODE, bool BATCH, bool PACKET, bool PERSIST>
 hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                           uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                          bool stats, uint32_t tile_offset, unsigned int* work, uint32_t bpc) {
+                          bool stats, uint32_t tile_offset, unsigned int* work, uint32_t bpc,
+                          const uint32_t* rects) {
   dim3 grid((total_tiles - tile_offset + 3u) / 4u);
   if (PERSIST) {
     hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(GT_WORK_WORDS), 0, st, work);
@@ -1007,7 +1062,7 @@ hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt
   const dim3 block(GT_BLOCK);
   const size_t lds = PACKET ? (size_t)(GT_BLOCK / 64) * 3u * GT_STACK_MAX * sizeof(uint32_t)
                             : (size_t)lds_depth * GT_BLOCK * sizeof(uint32_t);
-#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset, work
+#define GT_ARGS S, F, cams, n_cams, total_tiles, tile_w, lds_depth, tile_offset, work, rects
   if (dump && stats)
     hipLaunchKernelGGL((k_forward<MAXK, MODE, true, true, BATCH, PACKET, PERSIST>), grid, block, lds, st, GT_ARGS);
   else if (dump)
@@ -1025,15 +1080,16 @@ hipError_t launch_tiles_p(const gt_dev_scene& S, const gt_dev_frame& F, const gt
 template <int MAXK, int MODE, bool BATCH>
 hipError_t launch_tiles(const gt_dev_scene& S, const gt_dev_frame& F, const gt_dev_frame* cams, uint32_t n_cams,
                         uint32_t total_tiles, uint32_t tile_w, uint32_t lds_depth, hipStream_t st, bool dump,
-                        bool stats, uint32_t tile_offset, uint32_t path, unsigned int* work, uint32_t bpc) {
+                        bool stats, uint32_t tile_offset, uint32_t path, unsigned int* work, uint32_t bpc,
+                        const uint32_t* rects) {
   if (path == 1)
     return launch_tiles_p<MAXK, MODE, BATCH, false, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
-                                                           dump, stats, tile_offset, work, bpc);
+                                                           dump, stats, tile_offset, work, bpc, nullptr);
   if (path == 4)
     return launch_tiles_p<MAXK, MODE, BATCH, true, true>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st,
-                                                         dump, stats, tile_offset, work, bpc);
+                                                         dump, stats, tile_offset, work, bpc, rects);
   return launch_tiles_p<MAXK, MODE, BATCH, true, false>(S, F, cams, n_cams, total_tiles, tile_w, lds_depth, st, dump,
-                                                        stats, tile_offset, work, bpc);
+                                                        stats, tile_offset, work, bpc, rects);
 }
 
 template <int MAXK, bool BATCH>
@@ -1063,6 +1119,23 @@ int device_cus() {
 
 std::mutex g_init_mu;
 bool g_faces_ready[64];
+
+/* The frame's proxy rectangles (gt_rect.h), GT_LEAF_MAX "empty" entries past
+ * the last proxy so that a leaf's eight entries are read without a bound. */
+__global__ __launch_bounds__(256) void k_rects(const float* __restrict__ prox, uint32_t n, const gt_dev_frame F,
+                                               uint2* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n + GT_LEAF_MAX) return;
+  uint32_t r[2] = {GT_RECT_EMPTY_LO, GT_RECT_EMPTY_HI};
+  if (i < n) {
+    const float4* P = reinterpret_cast<const float4*>(prox) + 4 * (size_t)i;
+    const float4 a = P[0], b = P[1], c = P[2];
+    const float w[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    gt_proxy_rect(w, F.to_world, F.ax, F.ay, F.width, F.height, F.t_min, r);
+  }
+  out[i] = make_uint2(r[0], r[1]);
+}
+
 
 hipError_t ensure_faces() {
   int dev = 0;
@@ -1136,7 +1209,7 @@ extern "C" int gt_launch_colors(const gt_dev_scene* S, const gt_dev_frame* F, fl
 
 extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, const gt_dev_frame* cams_dev,
                                  uint32_t n_cams, uint32_t total_tiles, const gt_tuning* tune, void* stream,
-                                 uint32_t tile_offset, unsigned int* work) {
+                                 uint32_t tile_offset, unsigned int* work, uint32_t* rect_buf) {
   if (tile_offset >= total_tiles) return GT_OK;
   hipError_t e = ensure_faces();
   if (e != hipSuccess) {
@@ -1164,17 +1237,30 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
+  /* one camera, packet traversal: the frame's proxy rectangles first */
+  const uint32_t* rects = nullptr;
+  if (rect_buf && !batch && S->n_prox > 0 && path != 1) {
+    const uint32_t n = S->n_prox + GT_LEAF_MAX;
+    hipLaunchKernelGGL(k_rects, dim3((n + 255u) / 256u), dim3(256), 0, st, S->prox, S->n_prox, *F,
+                       reinterpret_cast<uint2*>(rect_buf));
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+      gt_set_error("proxy rectangle launch failed: %s", hipGetErrorString(e));
+      return GT_E_HIP;
+    }
+    rects = rect_buf;
+  }
 #define GT_TILES(MK, MODE)                                                                                       \
   (batch ? launch_tiles<MK, MODE, true>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                       tile_offset, path, work, bpc)                                             \
+                                       tile_offset, path, work, bpc, nullptr)                                    \
          : launch_tiles<MK, MODE, false>(*S, *F, cams_dev, n_cams, total_tiles, tile_w, depth, st, dump, stats, \
-                                        tile_offset, path, work, bpc))
+                                        tile_offset, path, work, bpc, rects))
   if (F->debug_mode == 1) {
     e = launch_tiles<1, MODE_CLOSEST, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                             tile_offset, path, work, bpc);
+                                             tile_offset, path, work, bpc, rects);
   } else if (F->debug_mode == 2) {
     e = launch_tiles<1, MODE_COUNT, false>(*S, *F, cams_dev, 1, total_tiles, tile_w, depth, st, dump, stats,
-                                           tile_offset, path, work, bpc);
+                                           tile_offset, path, work, bpc, rects);
   } else if (F->k <= 4) {
     e = GT_TILES(4, MODE_RENDER);
   } else if (F->k <= 8) {

@@ -107,10 +107,14 @@ extern "C" {
  * tile shape); total_tiles = all cameras' tiles.  Tuning path 4 renders the
  * same tiles with the persistent packet kernel, whose queue counters are
  * `work` (GT_WORK_WORDS device words, reset by the launch; required for
- * path 4, unused otherwise). */
+ * path 4, unused otherwise).  rect_buf: NULL, or device memory for
+ * GT_RECT_ENTRIES(n_prox) 8-B entries that a one-camera launch of the
+ * packet traversal fills with the frame's proxy rectangles (gt_rect.h) and
+ * culls with; results never depend on it. */
+#define GT_RECT_ENTRIES(n_prox) ((size_t)(n_prox) + GT_LEAF_MAX)
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
                       uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,
-                      unsigned int* work);
+                      unsigned int* work, uint32_t* rect_buf);
 /* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
  * array of n_cams frames (tile_base set, ascending); cam0: host copy of
  * frame 0, used directly when n_cams == 1.  work: one device counter the

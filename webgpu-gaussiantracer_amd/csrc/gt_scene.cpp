@@ -379,17 +379,46 @@ uint32_t gt_frame_tiles(const gt_tuning* t, gt_dev_frame* f, uint32_t tile_base)
   return f->tiles_x * ((f->height + th - 1u) / th);
 }
 
-/* The work counters of `stream` in this scene (assigned on the stream's
- * first render), or NULL once GT_WORK_SLOTS streams hold one. */
+/* The slot of `stream` in this scene (assigned on the stream's first
+ * render), or -1 once GT_WORK_SLOTS streams hold one.  Caller holds ctl_mu. */
+static int stream_slot(gt_scene* s, void* stream) {
+  for (uint32_t i = 0; i < s->ctl_used; ++i)
+    if (s->ctl_streams[i] == stream) return (int)i;
+  if (s->ctl_used == GT_WORK_SLOTS) return -1;
+  s->ctl_streams[s->ctl_used] = stream;
+  return (int)s->ctl_used++;
+}
+
+/* The work counters of `stream` in this scene, or NULL once GT_WORK_SLOTS
+ * streams hold a slot. */
 unsigned int* gt_scene_work(gt_scene* s, void* stream) {
   std::lock_guard<std::mutex> lk(s->ctl_mu);
   unsigned int* base = (unsigned int*)s->ctl.p;
   if (!base) return nullptr;
-  for (uint32_t i = 0; i < s->ctl_used; ++i)
-    if (s->ctl_streams[i] == stream) return base + (size_t)i * GT_WORK_WORDS;
-  if (s->ctl_used == GT_WORK_SLOTS) return nullptr;
-  s->ctl_streams[s->ctl_used] = stream;
-  return base + (size_t)(s->ctl_used++) * GT_WORK_WORDS;
+  const int i = stream_slot(s, stream);
+  return i < 0 ? nullptr : base + (size_t)i * GT_WORK_WORDS;
+}
+
+/* The proxy-rectangle table of `stream` (GT_RECT_ENTRIES(n_prox) entries),
+ * or NULL: no slot, an allocation that fails, or one that would be needed
+ * while the stream captures a graph.  The rectangles only cull work, so a
+ * frame without them renders the same result. */
+uint32_t* gt_scene_rects(gt_scene* s, void* stream) {
+  std::lock_guard<std::mutex> lk(s->ctl_mu);
+  const int i = stream_slot(s, stream);
+  if (i < 0) return nullptr;
+  DevBuf& b = s->rects[i];
+  const size_t bytes = GT_RECT_ENTRIES(s->dev.n_prox) * 2 * sizeof(uint32_t);
+  if (!b.p || b.bytes < bytes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      return nullptr;
+    if (gt_devbuf_reserve(b, bytes) != GT_OK) {
+      (void)hipGetLastError(); /* the failed allocation is not the launch's error */
+      return nullptr;
+    }
+  }
+  return (uint32_t*)b.p;
 }
 
 /* The persistent packet kernel by default (tuning path 0 or 4); the tile
@@ -407,7 +436,7 @@ static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   if ((path == 2 || path == 4) && !work) t.path = 3;
   if (t.path != 2 || f->debug_mode != 0 || f->comp_ids || f->n_comp || f->antialias) {
     if (t.path == 2) t.path = 3;
-    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work);
+    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, gt_scene_rects(s, stream));
   }
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, work, (unsigned long long*)f->stats, &t, stream);
 }

@@ -46,6 +46,9 @@ struct gt_scene {
   void* ctl_streams[GT_WORK_SLOTS] = {};
   uint32_t ctl_used = 0;
   std::mutex ctl_mu;
+  /* the proxy rectangles of one-camera frames, one table per stream slot
+   * of ctl (sized on the stream's first render after a rebuild) */
+  DevBuf rects[GT_WORK_SLOTS];
   DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
   DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
   DevBuf aa;  /* antialiasing: the new frame before it is averaged in */
