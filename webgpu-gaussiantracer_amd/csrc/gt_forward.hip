@@ -369,15 +369,20 @@ __device__ void trace_packet(const gt_dev_scene& S, const RayState& r, float tmi
       const uint32_t cnt = ((ref >> GT_LEAF_COUNT_SHIFT) & GT_LEAF_COUNT_MASK) + 1u;
       /* uniform loop, lane predicate inside: a divergent loop around the
        * k-buffer updates made the compiler copy (and spill) the buffer */
-      /* one dword of each of the leaf's 64-B records, all requested before
-       * the first record is needed, so that the record loads of the later
-       * proxies hit the scalar cache instead of waiting on L2 one after
-       * another (the value is consumed after the loop) */
+      /* Without rectangles: one dword of each of the leaf's 64-B records,
+       * all requested before the first record is needed, so that the record
+       * loads of the later proxies hit the scalar cache instead of waiting
+       * on L2 one after another (the value is consumed after the loop).
+       * With rectangles most of them are culled and the loads do not pay. */
       typedef const __attribute__((address_space(4))) uint32_t cuint;
       cuint* P = as_const<cuint>(S.prox) + 16 * (size_t)first + 12;
       uint32_t touch[GT_LEAF_MAX];
 #pragma unroll
-      for (uint32_t q = 0; q < GT_LEAF_MAX; ++q) touch[q] = P[16 * (q < cnt ? q : cnt - 1u)];
+      for (uint32_t q = 0; q < GT_LEAF_MAX; ++q) touch[q] = 0;
+      if (!tr.rects) {
+#pragma unroll
+        for (uint32_t q = 0; q < GT_LEAF_MAX; ++q) touch[q] = P[16 * (q < cnt ? q : cnt - 1u)];
+      }
       /* The leaf's proxies whose screen rectangle meets the tile; the others
        * fail the bounding-sphere stage in every lane.  The leaf's eight
        * entries come in one 64-B scalar load (the table holds GT_LEAF_MAX
@@ -561,7 +566,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
   const uint32_t py = (lt / F.tiles_x) * tile_h + (lane / tile_w);
   const bool active = tile < total_tiles && px < F.width && py < F.height;
   TileRect tr;
-  tr.rects = rects;
+  tr.rects = BATCH ? nullptr : rects;
   {
     const uint32_t tx0 = (lt % F.tiles_x) * tile_w, ty0 = (lt / F.tiles_x) * tile_h;
     tr.lo = tx0 | ty0 << 16;
@@ -1125,7 +1130,7 @@ bool g_faces_ready[64];
 __global__ __launch_bounds__(256) void k_rects(const float* __restrict__ prox, uint32_t n, const gt_dev_frame F,
                                                uint2* __restrict__ out) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n + GT_LEAF_MAX) return;
+  if (i >= (uint32_t)GT_RECT_ENTRIES(n)) return;
   uint32_t r[2] = {GT_RECT_EMPTY_LO, GT_RECT_EMPTY_HI};
   if (i < n) {
     const float4* P = reinterpret_cast<const float4*>(prox) + 4 * (size_t)i;
@@ -1237,10 +1242,13 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
-  /* one camera, packet traversal: the frame's proxy rectangles first */
+  /* one camera, packet traversal: the frame's proxy rectangles first, when
+   * the frame has at least half as many rays as the scene has proxies (the
+   * pass costs the same for any frame size; in a 64-camera batch of views
+   * from outside the scene it cost more than it saved) */
   const uint32_t* rects = nullptr;
-  if (rect_buf && !batch && S->n_prox > 0 && path != 1) {
-    const uint32_t n = S->n_prox + GT_LEAF_MAX;
+  if (rect_buf && !batch && S->n_prox > 0 && path != 1 && 2ull * F->width * F->height >= S->n_prox) {
+    const uint32_t n = (uint32_t)GT_RECT_ENTRIES(S->n_prox);
     hipLaunchKernelGGL(k_rects, dim3((n + 255u) / 256u), dim3(256), 0, st, S->prox, S->n_prox, *F,
                        reinterpret_cast<uint2*>(rect_buf));
     e = hipGetLastError();

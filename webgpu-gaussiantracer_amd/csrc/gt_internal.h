@@ -109,8 +109,8 @@ extern "C" {
  * `work` (GT_WORK_WORDS device words, reset by the launch; required for
  * path 4, unused otherwise).  rect_buf: NULL, or device memory for
  * GT_RECT_ENTRIES(n_prox) 8-B entries that a one-camera launch of the
- * packet traversal fills with the frame's proxy rectangles (gt_rect.h) and
- * culls with; results never depend on it. */
+ * packet traversal may fill with the frame's proxy rectangles (gt_rect.h)
+ * and cull with; results never depend on it. */
 #define GT_RECT_ENTRIES(n_prox) ((size_t)(n_prox) + GT_LEAF_MAX)
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
                       uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,
