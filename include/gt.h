@@ -203,7 +203,11 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
  *                 wave slot: 5 for k <= 16 and 4 for k > 16 on path 4, 4 on path 2)
  *   tile_w        ray tile of a wave: tile_w x (64 / tile_w) pixels, tile_w in {4, 8, 16, 32}
  *   stack_lds     traversal stack entries per ray kept in LDS (8, 16, 32 or 64);
- *                 deeper entries spill to scratch memory */
+ *                 deeper entries spill to scratch memory
+ *   rects         one-camera frames of paths 3 and 4: 0 cull leaf proxies by
+ *                 per-frame screen rectangles when the frame has at least half
+ *                 as many rays as the scene has proxies, 1 for every frame,
+ *                 2 never */
 typedef struct gt_tuning {
   uint32_t path;
   uint32_t refill_min;
@@ -211,6 +215,7 @@ typedef struct gt_tuning {
   uint32_t blocks_per_cu;
   uint32_t tile_w;
   uint32_t stack_lds;
+  uint32_t rects;
 } gt_tuning;
 int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t);
 int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t);

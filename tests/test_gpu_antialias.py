@@ -23,8 +23,12 @@ def _print(*a):
 
 
 def _scene():
+    """The scene renders with screen-rectangle culling forced on (tuning
+    rects=1): the rectangles must hold every jittered ray."""
     raw = gt.generate_raw(n=20000, seed=0, config="c1")
-    return raw, gt.Scene(raw, alpha_min=0.1), orc.OracleScene(raw, 0.1)
+    s = gt.Scene(raw, alpha_min=0.1)
+    s.set_tuning(rects=1)
+    return raw, s, orc.OracleScene(raw, 0.1)
 
 
 def test_antialias_frames_parity():

@@ -44,7 +44,8 @@ int tune(gt_scene* s, uint32_t iters, hipStream_t st, gt_autotune_result* out, R
   gt_tuning best_t = saved;
   uint32_t tried = 0;
   int rc = GT_OK;
-  for (const gt_tuning& t : candidates()) {
+  for (gt_tuning t : candidates()) {
+    t.rects = saved.rects; /* the caller's choice of rectangle culling is kept */
     if ((rc = gt_scene_set_tuning(s, &t)) != GT_OK) break;
     if ((rc = render()) != GT_OK) break; /* warm-up, graph capture */
     hipEventRecord(e0, st);

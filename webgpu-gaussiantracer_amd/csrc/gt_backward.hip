@@ -528,6 +528,7 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
   tile8.path = 3;
   tile8.tile_w = 8;
   tile8.stack_lds = gt_scene_tuning(s)->stack_lds;
+  tile8.rects = gt_scene_tuning(s)->rects;
   (void)gt_frame_tiles(&tile8, &f, 0);
   const uint32_t tiles_y = (height + 7u) / 8u;
   const uint64_t budget = budget_bytes ? budget_bytes : (1ull << 30);
@@ -556,7 +557,8 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
     const uint32_t npix = (y1 - y0) * width;
     f.comp_base = y0 * width;
     rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x, nullptr,
-                           gt_scene_rects(s, stream));
+                           gt_rects_wanted(&tile8, f.width, f.height, S->n_prox) ? gt_scene_rects(s, stream)
+                                                                                 : nullptr);
     if (rc != GT_OK) break;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dmax);
     hipLaunchKernelGGL(k_max_u32, dim3((npix + 255) / 256 < 1024 ? (npix + 255) / 256 : 1024), dim3(256), 0, st,

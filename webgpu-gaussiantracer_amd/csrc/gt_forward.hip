@@ -1242,12 +1242,11 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
   uint32_t tile_w, depth;
   tuning_defaults(tune, &tile_w, &depth);
   hipStream_t st = (hipStream_t)stream;
-  /* one camera, packet traversal: the frame's proxy rectangles first, when
-   * the frame has at least half as many rays as the scene has proxies (the
-   * pass costs the same for any frame size; in a 64-camera batch of views
-   * from outside the scene it cost more than it saved) */
+  /* one camera, packet traversal: the frame's proxy rectangles first (not
+   * for batches: in a 64-camera batch of views from outside the scene one
+   * table per camera cost more than it culled) */
   const uint32_t* rects = nullptr;
-  if (rect_buf && !batch && S->n_prox > 0 && path != 1 && 2ull * F->width * F->height >= S->n_prox) {
+  if (rect_buf && !batch && path != 1 && gt_rects_wanted(tune, F->width, F->height, S->n_prox)) {
     const uint32_t n = (uint32_t)GT_RECT_ENTRIES(S->n_prox);
     hipLaunchKernelGGL(k_rects, dim3((n + 255u) / 256u), dim3(256), 0, st, S->prox, S->n_prox, *F,
                        reinterpret_cast<uint2*>(rect_buf));

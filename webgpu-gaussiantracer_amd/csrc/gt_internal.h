@@ -112,6 +112,15 @@ extern "C" {
  * packet traversal may fill with the frame's proxy rectangles (gt_rect.h)
  * and cull with; results never depend on it. */
 #define GT_RECT_ENTRIES(n_prox) ((size_t)(n_prox) + GT_LEAF_MAX)
+/* does a one-camera frame of width x height use rectangles (tuning field
+ * rects: 0 when the frame has at least half as many rays as the scene has
+ * proxies, since the pass costs the same at any frame size; 1 always; 2
+ * never)? */
+static inline int gt_rects_wanted(const gt_tuning* t, uint32_t width, uint32_t height, uint32_t n_prox) {
+  const uint32_t mode = t ? t->rects : 0u;
+  if (mode == 2u || n_prox == 0) return 0;
+  return mode == 1u || 2ull * width * height >= n_prox;
+}
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
                       uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,
                       unsigned int* work, uint32_t* rect_buf);

@@ -436,7 +436,8 @@ static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   if ((path == 2 || path == 4) && !work) t.path = 3;
   if (t.path != 2 || f->debug_mode != 0 || f->comp_ids || f->n_comp || f->antialias) {
     if (t.path == 2) t.path = 3;
-    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, gt_scene_rects(s, stream));
+    uint32_t* rects = gt_rects_wanted(&t, f->width, f->height, s->dev.n_prox) ? gt_scene_rects(s, stream) : nullptr;
+    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, rects);
   }
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, work, (unsigned long long*)f->stats, &t, stream);
 }
@@ -446,9 +447,11 @@ extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
   const bool tile_ok = t->tile_w == 0 || t->tile_w == 4 || t->tile_w == 8 || t->tile_w == 16 || t->tile_w == 32;
   const bool lds_ok = t->stack_lds == 0 || t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32 ||
                       t->stack_lds == 64;
-  if (t->path > 4 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok) {
+  if (t->path > 4 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok ||
+      t->rects > 2) {
     gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u, tile_w %u, "
-                 "stack_lds %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w, t->stack_lds);
+                 "stack_lds %u, rects %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w,
+                 t->stack_lds, t->rects);
     return GT_E_INVALID;
   }
   s->tune = *t;
