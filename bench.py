@@ -52,7 +52,7 @@ sys.path.insert(0, os.path.join(ROOT, "webgpu-gaussiantracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
-FWD_KERNEL = "k_forward<16, 0, false, false, false, true, true> (packet traversal, persistent kernel)"
+FWD_KERNEL = "k_forward<32, 0, false, false, false, true, true> (packet traversal, persistent kernel, 32-slot buffer serving rounds of k)"
 
 
 def _counter_csv_rows(outdir):

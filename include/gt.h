@@ -200,7 +200,7 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
  *   comp_min      path 2: lanes waiting to composite that trigger a
  *                 composite pass (1..64)
  *   blocks_per_cu paths 2 and 4: resident 256-thread blocks per CU (1..8; 0 = one per
- *                 wave slot: 5 for k <= 16 and 4 for k > 16 on path 4, 4 on path 2)
+ *                 wave slot: 4 on paths 2 and 4)
  *   tile_w        ray tile of a wave: tile_w x (64 / tile_w) pixels, tile_w in {4, 8, 16, 32}
  *   stack_lds     traversal stack entries per ray kept in LDS (8, 16, 32 or 64);
  *                 deeper entries spill to scratch memory
