@@ -42,10 +42,11 @@ __constant__ float c_faces[12 * GT_NUM_FACES];
 /* Waves per SIMD the register budget of k_forward is sized for (512 /
  * waves VGPRs per lane), by k-buffer size.  The packet kernel is bound by
  * per-wave latency (stack pop -> scalar node load -> box tests -> sort ->
- * push), so occupancy pays: at MAXK <= 16, five waves (96 VGPRs; the few
- * spills sit outside the traversal loop) run the C2 frame 5 % faster than
- * four (128 VGPRs, no spills), and six spill too much.  The 64 registers
- * of a 32-slot buffer spill heavily at five waves, so MAXK 32 keeps four. */
+ * push), so occupancy pays: with a 16-slot buffer five waves (96 VGPRs)
+ * ran the 1080p 1M-Gaussian frame 5 % faster than four, and six spilled
+ * too much.  Renders use the 32-slot buffer, whose 64 key registers spill
+ * 248 B per lane at five waves (12.8 against 8.0 ms), so it keeps four;
+ * the one-slot debug-mode kernels run five. */
 #ifndef GT_WAVES_K16
 #define GT_WAVES_K16 5
 #endif
