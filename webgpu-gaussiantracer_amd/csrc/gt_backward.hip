@@ -34,7 +34,7 @@
 /* gt_scene.cpp */
 void* gt_scene_bwd_scratch(gt_scene* s, int which, size_t bytes);
 uint32_t* gt_scene_rc_cap(gt_scene* s);
-uint32_t* gt_scene_rects(gt_scene* s, void* stream);
+uint32_t* gt_scene_rects(gt_scene* s, void* stream, size_t extra_words, size_t* words);
 const gt_dev_scene* gt_scene_dev(const gt_scene* s);
 const gt_tuning* gt_scene_tuning(const gt_scene* s);
 int gt_validate_uniforms(const gt_uniforms* u, uint32_t width, uint32_t height);
@@ -556,9 +556,11 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
     const uint32_t y0 = ty0 * 8u, y1 = ty1 * 8u < height ? ty1 * 8u : height;
     const uint32_t npix = (y1 - y0) * width;
     f.comp_base = y0 * width;
-    rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x, nullptr,
-                           gt_rects_wanted(&tile8, f.width, f.height, S->n_prox) ? gt_scene_rects(s, stream)
-                                                                                 : nullptr);
+    size_t rwords = 0;
+    uint32_t* rbuf = gt_rects_wanted(&tile8, f.width, f.height, S->n_prox) ? gt_scene_rects(s, stream, 0, &rwords)
+                                                                            : nullptr;
+    rc = gt_launch_forward(S, &f, nullptr, 1, ty1 * f.tiles_x, &tile8, stream, ty0 * f.tiles_x, nullptr, rbuf,
+                           rwords);
     if (rc != GT_OK) break;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dmax);
     hipLaunchKernelGGL(k_max_u32, dim3((npix + 255) / 256 < 1024 ? (npix + 255) / 256 : 1024), dim3(256), 0, st,

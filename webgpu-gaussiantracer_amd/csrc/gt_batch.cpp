@@ -105,7 +105,8 @@ static int launch(gt_batch* b, hipStream_t st) {
   if (t->path == 2)
     return gt_launch_persistent(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, work,
                                 (unsigned long long*)b->frames[0].stats, t, st);
-  return gt_launch_forward(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, t, st, 0, work, nullptr);
+  return gt_launch_forward(gt_scene_dev(b->scene), &b->frames[0], cams, n, b->total_tiles, t, st, 0, work, nullptr,
+                           0);
 }
 
 extern "C" int gt_batch_create(gt_scene* s, uint32_t n, const gt_uniforms* u, const uint32_t* widths,

@@ -107,11 +107,33 @@ extern "C" {
  * tile shape); total_tiles = all cameras' tiles.  Tuning path 4 renders the
  * same tiles with the persistent packet kernel, whose queue counters are
  * `work` (GT_WORK_WORDS device words, reset by the launch; required for
- * path 4, unused otherwise).  rect_buf: NULL, or device memory for
- * GT_RECT_ENTRIES(n_prox) 8-B entries that a one-camera launch of the
- * packet traversal may fill with the frame's proxy rectangles (gt_rect.h)
- * and cull with; results never depend on it. */
+ * path 4, unused otherwise).  rect_buf: NULL, or device memory of
+ * rect_words 32-bit words, at least 2 GT_RECT_ENTRIES(n_prox), that a
+ * one-camera launch of the packet traversal may fill with the frame's proxy
+ * rectangles (gt_rect.h) and cull with; with gt_order_words more, a
+ * persistent launch also orders its tile queue by the rectangles' coverage.
+ * Results never depend on either. */
 #define GT_RECT_ENTRIES(n_prox) ((size_t)(n_prox) + GT_LEAF_MAX)
+/* Cost-ordered tile queue of a one-camera persistent frame (gt_forward.hip).
+ * Coverage is counted on a grid of cells of cell x cell tiles, the
+ * smallest cell for which the grid fits GT_COVER_CELLS counters (one LDS
+ * grid per workgroup), by GT_COVER_PARTS workgroups into partial grids that
+ * are then summed.  The workspace (32-bit words, after the rectangles in
+ * rect_buf) holds the partial grids, the summed grid, GT_ORDER_BUCKETS
+ * bucket counters and the order (one word per tile). */
+#define GT_ORDER_BUCKETS (GT_XCDS * 1024u)
+#define GT_COVER_CELLS 12288u
+#define GT_COVER_PARTS 128u
+static inline uint32_t gt_cover_cell(uint32_t tiles_x, uint32_t tiles_y) {
+  uint32_t c = 1;
+  while ((size_t)((tiles_x + c - 1u) / c) * ((tiles_y + c - 1u) / c) > GT_COVER_CELLS) ++c;
+  return c;
+}
+static inline size_t gt_order_words(uint32_t tiles_x, uint32_t tiles_y) {
+  const uint32_t c = gt_cover_cell(tiles_x, tiles_y);
+  const size_t cells = (size_t)((tiles_x + c - 1u) / c) * ((tiles_y + c - 1u) / c);
+  return (GT_COVER_PARTS + 1u) * cells + GT_ORDER_BUCKETS + (size_t)tiles_x * tiles_y;
+}
 /* does a one-camera frame of width x height use rectangles (tuning field
  * rects: 0 when the frame has at least half as many rays as the scene has
  * proxies, since the pass costs the same at any frame size; 1 always; 2
@@ -123,7 +145,7 @@ static inline int gt_rects_wanted(const gt_tuning* t, uint32_t width, uint32_t h
 }
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
                       uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,
-                      unsigned int* work, uint32_t* rect_buf);
+                      unsigned int* work, uint32_t* rect_buf, size_t rect_words);
 /* Persistent renderer over n_cams cameras sharing k.  cams_dev: device
  * array of n_cams frames (tile_base set, ascending); cam0: host copy of
  * frame 0, used directly when n_cams == 1.  work: one device counter the

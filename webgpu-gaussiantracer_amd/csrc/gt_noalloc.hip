@@ -163,5 +163,5 @@ extern "C" int gt_batch_render_noalloc(gt_scene* s, uint32_t n, const gt_uniform
   if (t->path == 2)
     return gt_launch_persistent(gt_scene_dev(s), &first, table, n, total, work,
                                 (unsigned long long*)outs[0].stats, t, stream);
-  return gt_launch_forward(gt_scene_dev(s), &first, table, n, total, t, stream, 0, work, nullptr);
+  return gt_launch_forward(gt_scene_dev(s), &first, table, n, total, t, stream, 0, work, nullptr, 0);
 }

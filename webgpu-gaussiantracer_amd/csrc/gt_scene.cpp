@@ -399,16 +399,19 @@ unsigned int* gt_scene_work(gt_scene* s, void* stream) {
   return i < 0 ? nullptr : base + (size_t)i * GT_WORK_WORDS;
 }
 
-/* The proxy-rectangle table of `stream` (GT_RECT_ENTRIES(n_prox) entries),
- * or NULL: no slot, an allocation that fails, or one that would be needed
- * while the stream captures a graph.  The rectangles only cull work, so a
- * frame without them renders the same result. */
-uint32_t* gt_scene_rects(gt_scene* s, void* stream) {
+/* The proxy-rectangle table of `stream`: GT_RECT_ENTRIES(n_prox) entries
+ * and extra_words more (the tile-order workspace), its capacity in 32-bit
+ * words in *words; or NULL: no slot, an allocation that fails, or one that
+ * would be needed while the stream captures a graph.  The rectangles only
+ * cull work and order it, so a frame without them renders the same
+ * result. */
+uint32_t* gt_scene_rects(gt_scene* s, void* stream, size_t extra_words, size_t* words) {
+  *words = 0;
   std::lock_guard<std::mutex> lk(s->ctl_mu);
   const int i = stream_slot(s, stream);
   if (i < 0) return nullptr;
   DevBuf& b = s->rects[i];
-  const size_t bytes = GT_RECT_ENTRIES(s->dev.n_prox) * 2 * sizeof(uint32_t);
+  const size_t bytes = (GT_RECT_ENTRIES(s->dev.n_prox) * 2 + extra_words) * sizeof(uint32_t);
   if (!b.p || b.bytes < bytes) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing((hipStream_t)stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
@@ -418,6 +421,7 @@ uint32_t* gt_scene_rects(gt_scene* s, void* stream) {
       return nullptr;
     }
   }
+  *words = b.bytes / sizeof(uint32_t);
   return (uint32_t*)b.p;
 }
 
@@ -436,8 +440,11 @@ static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
   if ((path == 2 || path == 4) && !work) t.path = 3;
   if (t.path != 2 || f->debug_mode != 0 || f->comp_ids || f->n_comp || f->antialias) {
     if (t.path == 2) t.path = 3;
-    uint32_t* rects = gt_rects_wanted(&t, f->width, f->height, s->dev.n_prox) ? gt_scene_rects(s, stream) : nullptr;
-    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, rects);
+    size_t words = 0;
+    uint32_t* rects = gt_rects_wanted(&t, f->width, f->height, s->dev.n_prox)
+                          ? gt_scene_rects(s, stream, gt_order_words(f->tiles_x, tiles / f->tiles_x), &words)
+                          : nullptr;
+    return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, rects, words);
   }
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, work, (unsigned long long*)f->stats, &t, stream);
 }
