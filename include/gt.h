@@ -205,9 +205,10 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
  *   stack_lds     traversal stack entries per ray kept in LDS (8, 16, 32 or 64);
  *                 deeper entries spill to scratch memory
  *   rects         one-camera frames of paths 3 and 4: 0 cull leaf proxies by
- *                 per-frame screen rectangles when the frame has at least half
- *                 as many rays as the scene has proxies, 1 for every frame,
- *                 2 never */
+ *                 per-frame screen rectangles (and, on path 4, claim tiles in
+ *                 order of decreasing rectangle coverage) when the frame has
+ *                 at least a quarter as many rays as the scene has proxies,
+ *                 1 for every frame, 2 never */
 typedef struct gt_tuning {
   uint32_t path;
   uint32_t refill_min;

@@ -1144,18 +1144,13 @@ hipError_t launch_persistent(const gt_dev_scene& S, const gt_dev_frame& c0, cons
  * tile times).  Coverage is counted on a grid of cells of cell x cell tiles:
  * each of GT_COVER_PARTS workgroups adds the four corners of every
  * rectangle of its contiguous run of proxies (neighbours in the tree, so
- * few distinct cells) into a difference grid in LDS and stores it; the
- * partial grids are summed cell by cell and prefix-summed along rows and
- * columns.  No global atomics (device-scope atomics from eight XCDs on
- * the same cells cost 1.8 ms at 1M).  A counting sort on (XCD range,
- * log-scale coverage) buckets then orders each XCD's tile range by
- * decreasing coverage, within the range: the persistent kernel's ranges and
- * its stealing between them stay as they are; only the order in which
- * tiles are claimed changes. */
-__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = v;
-}
-
+ * few distinct cells) into a difference grid in LDS and stores it; one
+ * workgroup sums the partial grids, prefix-sums them along rows and
+ * columns and orders each XCD's tile range by decreasing coverage within
+ * the range.  No global atomics (device-scope atomics from eight XCDs on
+ * the same cells cost 1.8 ms at 1M).  The persistent kernel's ranges and its
+ * stealing between them stay as they are; only the order in which tiles
+ * are claimed changes. */
 /* cells cx x cy; a rectangle covering cells [a, b) x [c, d) adds +1 at
  * (c, a) and (d, b), -1 at (c, b) and (d, a); corners in row cy or column
  * cx reach no cell and are skipped */
@@ -1186,77 +1181,69 @@ __global__ __launch_bounds__(1024) void k_cover_part(const uint2* __restrict__ r
   for (uint32_t i = threadIdx.x; i < cells; i += 1024u) out[i] = g[i];
 }
 
-__global__ __launch_bounds__(256) void k_cover_sum(const int* __restrict__ part, uint32_t cells, uint32_t parts,
-                                                   int* __restrict__ grid) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= cells) return;
-  int s = 0;
-  for (uint32_t p = 0; p < parts; ++p) s += part[(size_t)p * cells + i];
-  grid[i] = s;
-}
-
-/* one block: prefix sums along the rows, then along the columns */
-__global__ __launch_bounds__(1024) void k_cover_prefix(int* __restrict__ grid, uint32_t gx, uint32_t gy) {
-  for (uint32_t y = threadIdx.x; y < gy; y += 1024u) {
+/* One workgroup: the partial grids summed into LDS, prefix sums along
+ * rows and columns there, then a counting sort of the tiles on (XCD range,
+ * log-scale coverage) buckets: histogram, exclusive scan, scatter, all in
+ * LDS.  Tile i belongs to XCD range x = min(7, (8 i + 7) / n), the largest
+ * x with n x / 8 <= i, the persistent kernel's partition. */
+__global__ __launch_bounds__(1024) void k_order(const int* __restrict__ part, uint32_t cx, uint32_t cy,
+                                                uint32_t cell, uint32_t tiles_x, uint32_t n,
+                                                uint32_t* __restrict__ order) {
+  __shared__ int g[GT_COVER_CELLS];
+  __shared__ uint32_t h[GT_ORDER_BUCKETS];
+  const uint32_t cells = cx * cy, t = threadIdx.x;
+  for (uint32_t i = t; i < cells; i += 1024u) {
+    int v = 0;
+    for (uint32_t p = 0; p < GT_COVER_PARTS; ++p) v += part[(size_t)p * cells + i];
+    g[i] = v;
+  }
+  for (uint32_t i = t; i < GT_ORDER_BUCKETS; i += 1024u) h[i] = 0u;
+  __syncthreads();
+  for (uint32_t y = t; y < cy; y += 1024u) {
     int acc = 0;
-    for (uint32_t x = 0; x < gx; ++x) grid[(size_t)y * gx + x] = (acc += grid[(size_t)y * gx + x]);
+    for (uint32_t x = 0; x < cx; ++x) g[y * cx + x] = (acc += g[y * cx + x]);
   }
   __syncthreads();
-  for (uint32_t x = threadIdx.x; x < gx; x += 1024u) {
+  for (uint32_t x = t; x < cx; x += 1024u) {
     int acc = 0;
-    for (uint32_t y = 0; y < gy; ++y) grid[(size_t)y * gx + x] = (acc += grid[(size_t)y * gx + x]);
+    for (uint32_t y = 0; y < cy; ++y) g[y * cx + x] = (acc += g[y * cx + x]);
   }
-}
-
-/* the bucket of tile i: its XCD range (the persistent kernel's partition
- * lo(x) = n x / GT_XCDS), then decreasing coverage on a log scale */
-__device__ __forceinline__ uint32_t tile_bucket(const int* grid, uint32_t cx, uint32_t cell, uint32_t tiles_x,
-                                                uint32_t n, uint32_t i) {
-  uint32_t x = GT_XCDS - 1u;
-  while (x > 0 && (uint32_t)((uint64_t)n * x / GT_XCDS) > i) --x;
-  const int cov = grid[(size_t)((i / tiles_x) / cell) * cx + (i % tiles_x) / cell];
-  const float l = log2f((float)(cov > 0 ? cov : 0) + 1.0f) * 64.0f;
-  const uint32_t q = l < 1023.0f ? (uint32_t)l : 1023u;
-  return x * 1024u + (1023u - q);
-}
-
-__global__ __launch_bounds__(256) void k_tile_hist(const int* __restrict__ grid, uint32_t cx, uint32_t cell,
-                                                   uint32_t tiles_x, uint32_t n, uint32_t* __restrict__ hist) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) atomicAdd(hist + tile_bucket(grid, cx, cell, tiles_x, n, i), 1u);
-}
-
-/* one block: exclusive scan of the GT_ORDER_BUCKETS counters in place */
-__global__ __launch_bounds__(1024) void k_bucket_scan(uint32_t* __restrict__ hist) {
-  constexpr uint32_t PER = GT_ORDER_BUCKETS / 1024u;
-  __shared__ uint32_t part[1024];
-  uint32_t v[PER], sum = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < PER; ++j) {
-    v[j] = hist[threadIdx.x * PER + j];
-    sum += v[j];
-  }
-  part[threadIdx.x] = sum;
   __syncthreads();
-  for (uint32_t off = 1; off < 1024u; off <<= 1) { /* inclusive scan of the partial sums */
-    const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += add;
-    __syncthreads();
-  }
-  uint32_t base = part[threadIdx.x] - sum;
+  auto bucket = [&](uint32_t i) {
+    const uint32_t r = (8u * i + 7u) / n;
+    const int cov = g[((i / tiles_x) / cell) * cx + (i % tiles_x) / cell];
+    const float l = log2f((float)(cov > 0 ? cov : 0) + 1.0f) * (float)(GT_ORDER_LEVELS / 16u);
+    const uint32_t q = l < (float)(GT_ORDER_LEVELS - 1u) ? (uint32_t)l : GT_ORDER_LEVELS - 1u;
+    return (r < GT_XCDS ? r : GT_XCDS - 1u) * GT_ORDER_LEVELS + (GT_ORDER_LEVELS - 1u - q);
+  };
+  for (uint32_t i = t; i < n; i += 1024u) atomicAdd(&h[bucket(i)], 1u);
+  __syncthreads();
+  { /* exclusive scan of the buckets: PER per thread, then the partial sums */
+    constexpr uint32_t PER = GT_ORDER_BUCKETS / 1024u;
+    __shared__ uint32_t ps[1024];
+    uint32_t v[PER], sum = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < PER; ++j) {
-    hist[threadIdx.x * PER + j] = base;
-    base += v[j];
+    for (uint32_t j = 0; j < PER; ++j) {
+      v[j] = h[t * PER + j];
+      sum += v[j];
+    }
+    ps[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+      const uint32_t add = t >= off ? ps[t - off] : 0u;
+      __syncthreads();
+      ps[t] += add;
+      __syncthreads();
+    }
+    uint32_t base = ps[t] - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < PER; ++j) {
+      h[t * PER + j] = base;
+      base += v[j];
+    }
   }
-}
-
-__global__ __launch_bounds__(256) void k_tile_scatter(const int* __restrict__ grid, uint32_t cx, uint32_t cell,
-                                                      uint32_t tiles_x, uint32_t n, uint32_t* __restrict__ offs,
-                                                      uint32_t* __restrict__ order) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) order[atomicAdd(offs + tile_bucket(grid, cx, cell, tiles_x, n, i), 1u)] = i;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024u) order[atomicAdd(&h[bucket(i)], 1u)] = i;
 }
 
 /* ws: gt_order_words(tiles_x, tiles_y) words; returns the order in *order */
@@ -1266,18 +1253,10 @@ hipError_t order_tiles(uint32_t n_prox, const gt_dev_frame& F, uint32_t tile_w, 
   const uint32_t cell = gt_cover_cell(tiles_x, tiles_y);
   const uint32_t cx = (tiles_x + cell - 1u) / cell, cy = (tiles_y + cell - 1u) / cell, cells = cx * cy;
   int* part = reinterpret_cast<int*>(ws);
-  int* grid = part + (size_t)GT_COVER_PARTS * cells;
-  uint32_t* hist = reinterpret_cast<uint32_t*>(grid + cells);
-  uint32_t* ord = hist + GT_ORDER_BUCKETS;
+  uint32_t* ord = ws + (size_t)GT_COVER_PARTS * cells;
   hipLaunchKernelGGL(k_cover_part, dim3(GT_COVER_PARTS), dim3(1024), 0, st, rects, n_prox, F.width, F.height,
                      tile_w * cell, (64u / tile_w) * cell, cx, cy, part);
-  hipLaunchKernelGGL(k_cover_sum, dim3((cells + 255u) / 256u), dim3(256), 0, st, part, cells, GT_COVER_PARTS, grid);
-  hipLaunchKernelGGL(k_cover_prefix, dim3(1), dim3(1024), 0, st, grid, cx, cy);
-  hipLaunchKernelGGL(k_fill_u32, dim3(GT_ORDER_BUCKETS / 256u), dim3(256), 0, st, hist, GT_ORDER_BUCKETS, 0u);
-  hipLaunchKernelGGL(k_tile_hist, dim3((n + 255u) / 256u), dim3(256), 0, st, grid, cx, cell, tiles_x, n, hist);
-  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, hist);
-  hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255u) / 256u), dim3(256), 0, st, grid, cx, cell, tiles_x, n, hist,
-                     ord);
+  hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, part, cx, cy, cell, tiles_x, n, ord);
   *order = ord;
   return hipGetLastError();
 }

@@ -116,14 +116,14 @@ extern "C" {
 #define GT_RECT_ENTRIES(n_prox) ((size_t)(n_prox) + GT_LEAF_MAX)
 /* Cost-ordered tile queue of a one-camera persistent frame (gt_forward.hip).
  * Coverage is counted on a grid of cells of cell x cell tiles, the
- * smallest cell for which the grid fits GT_COVER_CELLS counters (one LDS
- * grid per workgroup), by GT_COVER_PARTS workgroups into partial grids that
- * are then summed.  The workspace (32-bit words, after the rectangles in
- * rect_buf) holds the partial grids, the summed grid, GT_ORDER_BUCKETS
- * bucket counters and the order (one word per tile). */
-#define GT_ORDER_BUCKETS (GT_XCDS * 1024u)
-#define GT_COVER_CELLS 12288u
-#define GT_COVER_PARTS 128u
+ * smallest cell for which the grid fits GT_COVER_CELLS counters (LDS grids),
+ * by GT_COVER_PARTS workgroups into partial grids that one workgroup then
+ * sums and orders.  The workspace (32-bit words, after the rectangles in
+ * rect_buf) holds the partial grids and the order (one word per tile). */
+#define GT_ORDER_LEVELS 512u
+#define GT_ORDER_BUCKETS (GT_XCDS * GT_ORDER_LEVELS)
+#define GT_COVER_CELLS 8192u
+#define GT_COVER_PARTS 64u
 static inline uint32_t gt_cover_cell(uint32_t tiles_x, uint32_t tiles_y) {
   uint32_t c = 1;
   while ((size_t)((tiles_x + c - 1u) / c) * ((tiles_y + c - 1u) / c) > GT_COVER_CELLS) ++c;
@@ -132,16 +132,19 @@ static inline uint32_t gt_cover_cell(uint32_t tiles_x, uint32_t tiles_y) {
 static inline size_t gt_order_words(uint32_t tiles_x, uint32_t tiles_y) {
   const uint32_t c = gt_cover_cell(tiles_x, tiles_y);
   const size_t cells = (size_t)((tiles_x + c - 1u) / c) * ((tiles_y + c - 1u) / c);
-  return (GT_COVER_PARTS + 1u) * cells + GT_ORDER_BUCKETS + (size_t)tiles_x * tiles_y;
+  return GT_COVER_PARTS * cells + (size_t)tiles_x * tiles_y;
 }
 /* does a one-camera frame of width x height use rectangles (tuning field
- * rects: 0 when the frame has at least half as many rays as the scene has
- * proxies, since the pass costs the same at any frame size; 1 always; 2
- * never)? */
+ * rects: 0 when the frame has at least a quarter as many rays as the scene
+ * has proxies, 1 always, 2 never)?  Measured with the tile ordering, views
+ * from outside the scene: 1280x720 gains 5-27 % at 1M and 3M proxies,
+ * 640x480 is even at 1M, and smaller frames lose 8-28 % (few tiles, little
+ * to cull, and the leaf-record prefetch that rectangles turn off matters
+ * most to waves that run alone). */
 static inline int gt_rects_wanted(const gt_tuning* t, uint32_t width, uint32_t height, uint32_t n_prox) {
   const uint32_t mode = t ? t->rects : 0u;
   if (mode == 2u || n_prox == 0) return 0;
-  return mode == 1u || 2ull * width * height >= n_prox;
+  return mode == 1u || 4ull * width * height >= n_prox;
 }
 int gt_launch_forward(const gt_dev_scene* s, const gt_dev_frame* f, const gt_dev_frame* cams_dev, uint32_t n_cams,
                       uint32_t total_tiles, const gt_tuning* tune, void* stream, uint32_t tile_offset,

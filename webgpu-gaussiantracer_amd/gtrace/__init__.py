@@ -352,7 +352,7 @@ class Scene:
     def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0, rects=0):
         """Launch tuning (results do not depend on it); 0 = library default.
         rects: screen-rectangle culling of one-camera frames, 0 when the frame
-        has at least half as many rays as the scene has proxies, 1 always,
+        has at least a quarter as many rays as the scene has proxies, 1 always,
         2 never."""
         _check(_lib.gt_scene_set_tuning(self._h, C.byref(Tuning(path, refill_min, comp_min, blocks_per_cu, tile_w,
                                                                 stack_lds, rects))))
