@@ -398,9 +398,11 @@ def backward_counts(torch, cids, ncomp, W, H, n):
     return out
 
 
-def backward_bytes(c, R, n):
+def backward_bytes(c, R, n, W, H):
     """Algorithmic bytes of one checkpoint-mode gt_backward call by kernel."""
+    nb = ((W + 15) // 16) * ((H + 15) // 16)  # 16x16-pixel blocks
     b = {"k_zero_f32": 64 * n, "k_colors": 224 * n,
+         "k_bw_cost": 4 * R + 4 * nb, "k_bw_order": 12 * nb,  # counts; block costs; order
          "k_backward": 4 * R + 16 * c["R_hit"] + 136 * c["H_comp"] + 112 * c["P"],
          "k_finalize": 64 * n + 712 * c["N_touched"]}
     b["total"] = sum(b.values())
@@ -480,7 +482,7 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
         ms_rc = _events_ms(torch, stream, step_rc, rc_steps)
     h_comp, h_eval = st_fwd["h_comp"], st_fwd["h_eval"]
     cnt = backward_counts(torch, cids, ncomp, W, H, n)
-    B = backward_bytes(cnt, R, n)
+    B = backward_bytes(cnt, R, n, W, H)
     B_fwd_ck = 16 * R + 44 * h_eval + 192 * h_comp + 4 * h_comp + 4 * R  # + the checkpoint's ids and counts
     B_loss = 80 * R + 236 * n  # dL/dRGBA = 2 (render - target) and zeroing the N x 59 gradient
     B_step = B_fwd_ck + B_loss + B["total"]
@@ -499,7 +501,8 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
                        "+ 16 R_hit (dL/dRGBA) + 136 H_comp (id twice, 48-B record and 16-B colour in each of two "
                        "passes) + 112 P (one 56-B read-modify-write of the accumulators per distinct (16x16 block, "
                        "Gaussian) pair); k_finalize 64 N (accumulators) + 712 N_touched (record 48, SH 192, "
-                       "236-B gradient row read and written); k_zero_f32 64 N",
+                       "236-B gradient row read and written); k_zero_f32 64 N; k_bw_cost 4 R + 4 B and k_bw_order 12 B "
+                       "(the blocks' cost-first dispatch order; B = 16x16-pixel blocks)",
     }
     return {
         "workload": f"loss = sum (render - target)^2, target from means + N(0, 0.01); forward with checkpoint, "
