@@ -477,20 +477,35 @@ __device__ __forceinline__ void finalize_one(const gt_dev_scene& S, const gt_dev
   for (int k = 0; k < 4; ++k) out[55 + k] += (float)((dq[k] - g.q[k] * qd) / g.qn);
 }
 
-/* One thread per Gaussian, FIN_BLOCK Gaussians per block.  The block's
- * gradient rows (FIN_BLOCK x 236 B, contiguous) are copied to LDS with
- * coalesced loads, updated there, and written back the same way; a thread
- * updating its own row in HBM made each access instruction touch 64 rows
- * 236 B apart.  Blocks whose Gaussians have no accumulators skip the copy.
- * (Staging the SH records in LDS as well halved the blocks per CU and
- * doubled the time.) */
+/* One thread per Gaussian, FIN_BLOCK Gaussians per block.  The gradient
+ * rows of the block's Gaussians that have accumulators (a quarter of the
+ * 1M-Gaussian scene's at 1080p) are copied to LDS, updated there and
+ * written back; the other rows are neither read nor written.  Each wave
+ * copies its own rows as one flat run of 59-float rows, 64 consecutive
+ * floats per access instruction (two or three rows), so the accesses stay
+ * coalesced; a thread updating its own row in HBM made each access
+ * instruction touch 64 rows 236 B apart.  (Copying every row of a block
+ * with any accumulator moved 0.47 GB instead of 0.13 GB.  Staging the SH
+ * records in LDS as well halved the blocks per CU and doubled the time.) */
 #define FIN_BLOCK 128u
+__device__ __forceinline__ void fin_copy_rows(float* dst, const float* src, const uint8_t* rows, uint32_t nrows,
+                                              uint32_t lane) {
+  const uint32_t nf = nrows * GT_RAW_FLOATS;
+#pragma unroll 4
+  for (uint32_t j = lane; j < nf; j += 64u) {
+    const uint32_t r = j / GT_RAW_FLOATS;
+    const uint32_t e = (uint32_t)rows[r] * GT_RAW_FLOATS + (j - r * GT_RAW_FLOATS);
+    dst[e] = src[e];
+  }
+}
 __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const gt_dev_scene S, const gt_dev_frame F,
                                                         const float* __restrict__ acc, float* __restrict__ grads) {
   __shared__ float s_g[FIN_BLOCK * GT_RAW_FLOATS];
+  __shared__ uint8_t s_rows[FIN_BLOCK];
   const uint32_t base = blockIdx.x * FIN_BLOCK;
   const uint32_t cnt = S.n - base < FIN_BLOCK ? S.n - base : FIN_BLOCK;
   const uint32_t id = base + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, w0 = threadIdx.x & ~63u;
   float ac[GT_ACC] = {};
   bool any = false;
   if (threadIdx.x < cnt) {
@@ -503,14 +518,18 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finalize(const gt_dev_scene S, co
       any |= v[i] != 0.0f;
     }
   }
-  if (!__syncthreads_or(any)) return;
+  /* this wave's rows with accumulators, compacted in lane order */
+  const uint64_t m = __ballot(any);
+  if (!m) return;
+  const uint32_t nrows = (uint32_t)__popcll(m);
+  if (any) s_rows[w0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint8_t)threadIdx.x;
+  __builtin_amdgcn_wave_barrier();
   float* const gblk = grads + (size_t)base * GT_RAW_FLOATS;
-  const uint32_t nf = cnt * GT_RAW_FLOATS;
-  for (uint32_t j = threadIdx.x; j < nf; j += FIN_BLOCK) s_g[j] = gblk[j];
-  __syncthreads();
+  fin_copy_rows(s_g, gblk, s_rows + w0, nrows, lane);
+  __builtin_amdgcn_wave_barrier();
   if (any) finalize_one(S, F, id, ac, s_g + threadIdx.x * GT_RAW_FLOATS);
-  __syncthreads();
-  for (uint32_t j = threadIdx.x; j < nf; j += FIN_BLOCK) gblk[j] = s_g[j];
+  __builtin_amdgcn_wave_barrier();
+  fin_copy_rows(gblk, s_g, s_rows + w0, nrows, lane);
 }
 
 __global__ void k_zero_u32(uint32_t* p) {
