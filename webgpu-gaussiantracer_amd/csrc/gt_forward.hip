@@ -46,7 +46,9 @@ __constant__ float c_faces[12 * GT_NUM_FACES];
  * ran the 1080p 1M-Gaussian frame 5 % faster than four, and six spilled
  * too much.  Renders use the 32-slot buffer, whose 64 key registers spill
  * 248 B per lane at five waves (12.8 against 8.0 ms), so it keeps four;
- * the one-slot debug-mode kernels run five. */
+ * at four it needs 161 VGPRs and spills 100 B per lane, yet three waves
+ * (no spills) took 7.31 against 6.90 ms.  The one-slot debug-mode kernels
+ * run five. */
 #ifndef GT_WAVES_K16
 #define GT_WAVES_K16 5
 #endif
