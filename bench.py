@@ -10,11 +10,14 @@ Inputs are resident in HBM before the timed region; the output image is
 written to device memory.  value = rays / time (Mrays/s).
 
 roofline: algorithmic HBM bytes per frame
-  B_fwd = 16 R + 44 H_eval + 192 H_comp
-(R rays, H_eval evaluated k-buffer entries, H_comp composited entries: the
-RGBA write, the 44-B mu/scale/rotation/opacity read per evaluated hit and
-the 192-B SH read per composited hit), divided by the kernel's average
-duration measured with HIP events on the launch stream.  roofline.traffic is
+  B_fwd = 16 R + 44 H_eval + 16 H_comp + 224 N
+(R rays, H_eval evaluated k-buffer entries, H_comp composited entries, N
+Gaussians: the RGBA write, the 44-B mu/scale/rotation/opacity read per
+evaluated hit, the 16-B colour read per composited hit, and the colour pass
+that reads each Gaussian's 16-B mu and 192-B SH record and writes its 16-B
+colour; a frame with fewer than N / 2 rays reads the 192-B SH record per
+composited hit instead, 16 R + 44 H_eval + 192 H_comp), divided by the
+frame's average duration measured with HIP events on the launch stream.  roofline.traffic is
 the kernel's FETCH_SIZE + WRITE_SIZE per launch, measured in the same run by
 rocprofv3 --pmc child processes and scaled by what the counters read for a
 plain 1 GiB device copy (null, with the reason, when rocprofv3 is missing or
@@ -52,7 +55,18 @@ sys.path.insert(0, os.path.join(ROOT, "webgpu-gaussiantracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s fwd and fwd+bwd, 1M Gaussians 1080p k=16; % of HBM roofline"
-FWD_KERNEL = "k_forward<32, 0, false, false, false, true, true> (packet traversal, persistent kernel, 32-slot buffer serving rounds of k)"
+FWD_KERNEL = ("k_forward<32, 3, false, false, false, true, true> (packet traversal, persistent kernel, 32-slot buffer "
+              "serving rounds of k, per-Gaussian colours precomputed by k_colors)")
+
+
+def forward_bytes(R, h_eval, h_comp, n):
+    """Algorithmic bytes of one frame: the composite reads a 16-B colour per
+    hit after a colour pass over the N Gaussians (224 B each) when the frame
+    has at least N / 2 rays (the library's default), else a 192-B SH record
+    per hit."""
+    if 2 * R >= n:
+        return 16 * R + 44 * h_eval + 16 * h_comp + 224 * n
+    return 16 * R + 44 * h_eval + 192 * h_comp
 
 
 def _counter_csv_rows(outdir):
@@ -483,7 +497,7 @@ def bench_fwd_bwd(gt, torch, dev, sp, raw, scene, u, W, H, steps, warmup, ms_fwd
     h_comp, h_eval = st_fwd["h_comp"], st_fwd["h_eval"]
     cnt = backward_counts(torch, cids, ncomp, W, H, n)
     B = backward_bytes(cnt, R, n, W, H)
-    B_fwd_ck = 16 * R + 44 * h_eval + 192 * h_comp + 4 * h_comp + 4 * R  # + the checkpoint's ids and counts
+    B_fwd_ck = forward_bytes(R, h_eval, h_comp, n) + 4 * h_comp + 4 * R  # + the checkpoint's ids and counts
     B_loss = 80 * R + 236 * n  # dL/dRGBA = 2 (render - target) and zeroing the N x 59 gradient
     B_step = B_fwd_ck + B_loss + B["total"]
 
@@ -822,7 +836,7 @@ def main():
     ms_step = ms_total / args.steps
     mrays = R / (ms_step * 1e-3) / 1e6
 
-    B_fwd = 16 * R + 44 * st["h_eval"] + 192 * st["h_comp"]
+    B_fwd = forward_bytes(R, st["h_eval"], st["h_comp"], raw.shape[0])
     achieved = B_fwd / (ms_step * 1e-3) / 1e9
     img = rgba.cpu().numpy()
     copy_gbs = measured_copy_gbs(torch, stream)

@@ -208,7 +208,13 @@ int gt_debug_face_test(const float* w2o, const float* o, const float* d, uint32_
  *                 per-frame screen rectangles (and, on path 4, claim tiles in
  *                 order of decreasing rectangle coverage) when the frame has
  *                 at least a quarter as many rays as the scene has proxies,
- *                 1 for every frame, 2 never */
+ *                 1 for every frame, 2 never
+ *   colors        one-camera rendering frames of paths 1, 3 and 4: 0 composite
+ *                 from per-Gaussian colours computed for the frame's origin
+ *                 before the traversal when the frame has at least half as
+ *                 many rays as the scene has Gaussians (else each composited
+ *                 hit evaluates its SH record), 1 for every frame, 2 never;
+ *                 the colours are the same FP32 values either way */
 typedef struct gt_tuning {
   uint32_t path;
   uint32_t refill_min;
@@ -217,6 +223,7 @@ typedef struct gt_tuning {
   uint32_t tile_w;
   uint32_t stack_lds;
   uint32_t rects;
+  uint32_t colors;
 } gt_tuning;
 int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t);
 int gt_scene_get_tuning(const gt_scene* s, gt_tuning* t);

@@ -20,24 +20,26 @@ def uniforms(pos, target, W, H, **kw):
 
 
 def render_rects_both(s, u, W, H, max_rounds):
-    """The frame rendered with screen-rectangle culling forced on (tuning
-    rects=1) and off (rects=2), the scene's tuning otherwise kept: outputs
-    identical bit for bit, the same hits and node visits, no more proxy tests
-    with the culling.  Returns the culled render."""
+    """The frame rendered with screen-rectangle culling and precomputed
+    per-Gaussian colours forced on (tuning rects=1, colors=1) and both off
+    (rects=2, colors=2: every composited hit evaluates its SH record), the
+    scene's tuning otherwise kept: outputs identical bit for bit, the same
+    hits and node visits, no more proxy tests with the culling.  Returns the
+    render with both on."""
     t0 = s.get_tuning()
     try:
-        s.set_tuning(**dict(t0, rects=1))
+        s.set_tuning(**dict(t0, rects=1, colors=1))
         g = s.render(u, W, H, max_rounds=max_rounds, want_stats=True)
-        s.set_tuning(**dict(t0, rects=2))
+        s.set_tuning(**dict(t0, rects=2, colors=2))
         g2 = s.render(u, W, H, max_rounds=max_rounds, want_stats=True)
     finally:
         s.set_tuning(**t0)
-    np.testing.assert_array_equal(g["rgba"].view(np.uint32), g2["rgba"].view(np.uint32), err_msg="rects: rgba")
+    np.testing.assert_array_equal(g["rgba"].view(np.uint32), g2["rgba"].view(np.uint32), err_msg="rects/colors: rgba")
     for key in ("hit_ids", "accept", "n_rounds"):
-        np.testing.assert_array_equal(g[key], g2[key], err_msg=f"rects: {key}")
-    np.testing.assert_array_equal(g["T"].view(np.uint32), g2["T"].view(np.uint32), err_msg="rects: T")
+        np.testing.assert_array_equal(g[key], g2[key], err_msg=f"rects/colors: {key}")
+    np.testing.assert_array_equal(g["T"].view(np.uint32), g2["T"].view(np.uint32), err_msg="rects/colors: T")
     for key in ("rays", "rounds", "h_eval", "h_comp", "hits", "nodes"):
-        assert g["stats"][key] == g2["stats"][key], f"rects: stats {key}"
+        assert g["stats"][key] == g2["stats"][key], f"rects/colors: stats {key}"
     assert g["stats"]["proxies"] <= g2["stats"]["proxies"]
     return g
 
@@ -45,8 +47,9 @@ def render_rects_both(s, u, W, H, max_rounds):
 def check_parity(s, o, u, W, H, max_rounds=24, label=""):
     """Kernel vs oracle: hit lists, accept masks, rounds and T bit-exact;
     pixels within PIXEL_TOL of the FP64 re-evaluation.  The kernel renders
-    with and without screen-rectangle culling (identical outputs required);
-    the culled render is checked.  Returns both results."""
+    with and without screen-rectangle culling and precomputed colours
+    (identical outputs required); the render with both is checked.  Returns
+    both results."""
     g = render_rects_both(s, u, W, H, max_rounds)
     p = orc.params_from_uniforms(u, W, H)
     r = orc.render(o, p, max_rounds=max_rounds)

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
-FWD = "void (anonymous namespace)::k_forward<32, 0, false, false, false, true, true>(gt_dev_scene, ...)"
+FWD = "void (anonymous namespace)::k_forward<32, 3, false, false, false, true, true>(gt_dev_scene, ...)"
 
 
 def _write_counters(path, rows):

@@ -23,11 +23,13 @@ def _print(*a):
 
 
 def _scene():
-    """The scene renders with screen-rectangle culling forced on (tuning
-    rects=1): the rectangles must hold every jittered ray."""
+    """The scene renders with screen-rectangle culling and precomputed
+    colours forced on (tuning rects=1, colors=1): the rectangles must hold
+    every jittered ray, and the colours of the camera origin serve every
+    jittered ray (the origin does not move)."""
     raw = gt.generate_raw(n=20000, seed=0, config="c1")
     s = gt.Scene(raw, alpha_min=0.1)
-    s.set_tuning(rects=1)
+    s.set_tuning(rects=1, colors=1)
     return raw, s, orc.OracleScene(raw, 0.1)
 
 

@@ -1,9 +1,11 @@
 """A one-camera render (device buffers) captured into a graph and replayed.
 
-The frame's proxy-rectangle table is allocated per stream on its first
-render outside a capture.  Captured on a stream that has not rendered yet,
-the render takes no rectangles (nothing may be allocated during a capture);
-captured on a stream that has, the rectangle pass is part of the graph.
+The frame's proxy-rectangle table and its per-Gaussian colour buffer are
+allocated per stream on its first render outside a capture.  Captured on a
+stream that has not rendered yet, the render takes neither (nothing may be
+allocated during a capture: each composited hit evaluates its SH record);
+captured on a stream that has, the rectangle and colour passes are part of
+the graph.
 Either way every replay equals the eager render bit for bit.  The replays
 run with eager work on the same and another stream between them, and the
 output is cleared by a kernel before each one.
@@ -25,7 +27,7 @@ def test_single_camera_render_replays_from_a_graph(warm):
     torch.cuda.set_device(dev)
     raw = gt.generate_raw(n=60000, seed=3, config="c1")
     s = gt.Scene(raw, alpha_min=0.05)
-    s.set_tuning(rects=1)  # the culling on for this small frame
+    s.set_tuning(rects=1, colors=1)  # the culling and the colour pass on for this small frame
     W, H = 160, 120
     u = uniforms([0.3, -0.2, 7.0], [0, 0, 0], W, H, hfov_deg=55, alpha_min=0.05, k=12, T_min=0.01)
     u_other = uniforms([-0.5, 0.4, 6.0], [0, 0, 0], W, H, hfov_deg=55, alpha_min=0.05, k=12, T_min=0.01)

@@ -45,7 +45,8 @@ int tune(gt_scene* s, uint32_t iters, hipStream_t st, gt_autotune_result* out, R
   uint32_t tried = 0;
   int rc = GT_OK;
   for (gt_tuning t : candidates()) {
-    t.rects = saved.rects; /* the caller's choice of rectangle culling is kept */
+    t.rects = saved.rects; /* the caller's choices of rectangle culling and colours are kept */
+    t.colors = saved.colors;
     if ((rc = gt_scene_set_tuning(s, &t)) != GT_OK) break;
     if ((rc = render()) != GT_OK) break; /* warm-up, graph capture */
     hipEventRecord(e0, st);

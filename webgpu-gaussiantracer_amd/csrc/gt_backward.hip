@@ -627,6 +627,10 @@ extern "C" int gt_backward(gt_scene* s, const gt_uniforms* u, uint32_t width, ui
   tile8.tile_w = 8;
   tile8.stack_lds = gt_scene_tuning(s)->stack_lds;
   tile8.rects = gt_scene_tuning(s)->rects;
+  tile8.colors = gt_scene_tuning(s)->colors;
+  /* the band forwards composite the colours computed above (the same
+   * values their SH evaluation would give) */
+  if (tile8.colors != 2u) f.colors = reinterpret_cast<const float*>(colors);
   (void)gt_frame_tiles(&tile8, &f, 0);
   const uint32_t tiles_y = (height + 7u) / 8u;
   const uint64_t budget = budget_bytes ? budget_bytes : (1ull << 30);

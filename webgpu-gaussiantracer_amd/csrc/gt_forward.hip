@@ -127,6 +127,17 @@ struct KBuf {
   __device__ __forceinline__ uint32_t id_at(int i) const { return (uint32_t)key[i]; }
   __device__ __forceinline__ float kth() const { return t_at(MAXK - 1); }
 
+  /* every slot the sentinel (t_max, GT_SENTINEL); t_max must be the same in
+   * every lane of the wave.  The opaque copy lives in a scalar register: the
+   * reset values cannot be hoisted out of the round loop, and no vector
+   * register holds the input across the traversal. */
+  __device__ __forceinline__ void reset_empty(float t_max) {
+    uint32_t tb = __float_as_uint(t_max);
+    asm volatile("" : "+s"(tb));
+    const uint64_t sentinel = ((uint64_t)tb << 32) | GT_SENTINEL;
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) key[i] = sentinel;
+  }
   __device__ __forceinline__ void reset(uint32_t off, float t_max) {
     /* opaque copies of the (uniform) inputs: the MAXK reset values are
      * recomputed in each round instead of hoisted out of the round loop and
@@ -164,7 +175,13 @@ struct Counters {
   uint32_t rays = 0, rounds = 0, h_eval = 0, h_comp = 0, proxies = 0, hits = 0, nodes = 0, overflow = 0;
 };
 
-enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2 };
+/* MODE_RENDER_PC renders as MODE_RENDER, compositing the frame's
+ * precomputed per-Gaussian colours (F.colors) instead of evaluating each
+ * composited hit's SH record: the same FP32 colour (k_colors runs the same
+ * radiance() on the same origin), without the 192-B record read and the
+ * basis per hit. */
+enum { MODE_RENDER = 0, MODE_CLOSEST = 1, MODE_COUNT = 2, MODE_RENDER_PC = 3 };
+__host__ __device__ constexpr bool is_render(int mode) { return mode == MODE_RENDER || mode == MODE_RENDER_PC; }
 
 /* Read-only scene arrays seen through the constant address space: a load
  * whose address is the same in every lane becomes one scalar load. */
@@ -601,7 +618,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
     const size_t MR = F.max_rounds;
     KBuf<MAXK> kb;
     uint32_t count = 0;
-    if (MODE == MODE_RENDER) {
+    if (is_render(MODE)) {
       const uint32_t k = F.k;
       float t_curr = F.t_min;
       uint32_t step = 0;
@@ -620,12 +637,12 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
        * state is added to the k-buffer's registers. */
       uint32_t rem = 0;
       uint64_t full = ~0ull;
-      kb.reset(0, F.t_max);
+      kb.reset_empty(F.t_max);
       while (F.T_min < T && t_curr < F.t_max) {
         if (F.early_stop == step++) break;
         const bool need = !((__builtin_amdgcn_inverse_ballot_w64(full) ? rem >= k : true) && kb.t_at(0) > t_curr);
         if (__ballot(need)) {
-          kb.reset(0, F.t_max);
+          kb.reset_empty(F.t_max);
           if (PACKET) trace_packet<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
           else trace<MAXK, MODE>(S, r, t_curr, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
           full = __ballot(kb.id_at(MAXK - 1) != GT_SENTINEL);
@@ -654,8 +671,13 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
               const gt_f3 mu = gt_f3_make(g0.x, g0.y, g0.z);
               const float alpha = (float)(gt_response64(mu3, sc3, q4, r.o, r.d) * (double)g0.w);
               if (F.alpha_min < alpha) {
-                const gt_f3 dir = gt_normalize(gt_sub(mu, r.o));
-                const gt_f3 c = radiance(S.sh + 48 * (size_t)id, F.sh_degree, dir);
+                gt_f3 c;
+                if constexpr (MODE == MODE_RENDER_PC) {
+                  const float4 c4 = reinterpret_cast<const float4*>(F.colors)[id];
+                  c = gt_f3_make(c4.x, c4.y, c4.z);
+                } else {
+                  c = radiance(S.sh + 48 * (size_t)id, F.sh_degree, gt_normalize(gt_sub(mu, r.o)));
+                }
                 const float Ta = T * alpha;
                 L = gt_add(L, gt_scale(Ta, c));
                 T *= 1.0f - alpha;
@@ -677,7 +699,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
       }
     } else if (MODE == MODE_CLOSEST) {
       /* debugMode 1 (GS_shader.ts:289-296, 312-321) */
-      kb.reset(0, F.t_max);
+      kb.reset_empty(F.t_max);
       if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
       else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t id = kb.id_at(MAXK - 1);
@@ -689,7 +711,7 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
       }
     } else {
       /* debugMode 2 (GS_shader.ts:201-228): front-face hits in (t_min, t_max) */
-      kb.reset(0, F.t_max);
+      kb.reset_empty(F.t_max);
       if (PACKET) trace_packet<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, wst, s_faces, tr);
       else trace<MAXK, MODE>(S, r, F.t_min, F.t_max, F.slack_abs, kb, count, st, stk, s_faces);
       const uint32_t c = count;
@@ -714,13 +736,13 @@ __device__ __forceinline__ void render_tile(const gt_dev_scene& S, const gt_dev_
     if (F.n_comp) F.n_comp[ray - F.comp_base] = n_comp;
     if (F.T_out) F.T_out[ray] = T;
     if (DUMP) {
-      const uint32_t kk = MODE == MODE_RENDER ? F.k : F.k;
-      const uint32_t first_unwritten = MODE == MODE_RENDER ? rounds : 1u;
+      const uint32_t kk = F.k;
+      const uint32_t first_unwritten = is_render(MODE) ? rounds : 1u;
       for (size_t rd = first_unwritten; rd < MR; ++rd) {
         for (uint32_t j = 0; j < kk; ++j) F.hit_ids[((size_t)ray * MR + rd) * kk + j] = GT_SENTINEL;
         F.accept[(size_t)ray * MR + rd] = 0;
       }
-      if (MODE != MODE_RENDER && MR > 0)
+      if (!is_render(MODE) && MR > 0)
         for (uint32_t j = 1; j < kk; ++j) F.hit_ids[(size_t)ray * MR * kk + j] = GT_SENTINEL;
     }
     st.rounds += rounds;
@@ -749,7 +771,11 @@ __global__ __launch_bounds__(GT_BLOCK, Waves<MAXK>::value) void k_forward(const 
   Stack stk;
   stk.lds = s_stack + threadIdx.x;
   stk.sp = 0;
-  uint32_t* const wst = s_stack + (threadIdx.x >> 6) * (3u * GT_STACK_MAX); /* PACKET: this wave's stack */
+  /* PACKET: this wave's stack.  The wave index goes through readfirstlane
+   * so that the base is a scalar: as a per-lane value the compiler kept it
+   * as a 64-bit pointer, spilled it, and reloaded it from scratch on every
+   * stack pop. */
+  uint32_t* const wst = s_stack + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * (3u * GT_STACK_MAX);
   stk.depth = (int)lds_depth;
   const uint32_t lane = threadIdx.x & 63u;
   Counters st;
@@ -1436,7 +1462,10 @@ extern "C" int gt_launch_forward(const gt_dev_scene* S, const gt_dev_frame* F, c
     /* every k renders with the 32-slot buffer: one traversal serves
      * several rounds of k (1080p 1M-Gaussian frame: k 16 8.28 -> 7.96 ms,
      * k 8 10.78 -> 8.45 ms against a k-slot buffer per round) */
-    e = GT_TILES(32, MODE_RENDER);
+    e = (F->colors && !batch) ? launch_tiles<32, MODE_RENDER_PC, false>(*S, *F, cams_dev, 1, total_tiles, tile_w,
+                                                                        depth, st, dump, stats, tile_offset, path,
+                                                                        work, bpc, rects, order)
+                              : GT_TILES(32, MODE_RENDER);
   }
 #undef GT_TILES
   if (e != hipSuccess) {

@@ -425,6 +425,28 @@ uint32_t* gt_scene_rects(gt_scene* s, void* stream, size_t extra_words, size_t* 
   return (uint32_t*)b.p;
 }
 
+/* This stream's buffer for the frame's per-Gaussian colours (n float4), or
+ * NULL: on a stream rendered from no slot, while the stream is being
+ * captured and the buffer would have to grow, or when it cannot be
+ * allocated.  The frame then evaluates SH per hit (the same values). */
+static float* scene_colors(gt_scene* s, void* stream) {
+  std::lock_guard<std::mutex> lk(s->ctl_mu);
+  const int i = stream_slot(s, stream);
+  if (i < 0) return nullptr;
+  DevBuf& b = s->colors[i];
+  const size_t bytes = (size_t)s->dev.n * 4u * sizeof(float);
+  if (!b.p || b.bytes < bytes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      return nullptr;
+    if (gt_devbuf_reserve(b, bytes) != GT_OK) {
+      (void)hipGetLastError(); /* the failed allocation is not the launch's error */
+      return nullptr;
+    }
+  }
+  return (float*)b.p;
+}
+
 /* The persistent packet kernel by default (tuning path 0 or 4); the tile
  * kernels with paths 1 and 3; the lane-refill persistent kernel with path 2
  * (which leaves debug modes, the backward checkpoint and antialiasing to the
@@ -444,6 +466,14 @@ static int launch_frame(gt_scene* s, gt_dev_frame* f, void* stream) {
     uint32_t* rects = gt_rects_wanted(&t, f->width, f->height, s->dev.n_prox)
                           ? gt_scene_rects(s, stream, gt_order_words(f->tiles_x, tiles / f->tiles_x), &words)
                           : nullptr;
+    if (f->debug_mode == 0 && gt_colors_wanted(&t, f->width, f->height, s->dev.n)) {
+      float* colors = scene_colors(s, stream);
+      if (colors) {
+        const int rc = gt_launch_colors(&s->dev, f, colors, stream);
+        if (rc != GT_OK) return rc;
+        f->colors = colors;
+      }
+    }
     return gt_launch_forward(&s->dev, f, nullptr, 1, tiles, &t, stream, 0, work, rects, words);
   }
   return gt_launch_persistent(&s->dev, f, nullptr, 1, tiles, work, (unsigned long long*)f->stats, &t, stream);
@@ -455,10 +485,10 @@ extern "C" int gt_scene_set_tuning(gt_scene* s, const gt_tuning* t) {
   const bool lds_ok = t->stack_lds == 0 || t->stack_lds == 8 || t->stack_lds == 16 || t->stack_lds == 32 ||
                       t->stack_lds == 64;
   if (t->path > 4 || t->refill_min > 64 || t->comp_min > 64 || t->blocks_per_cu > 8 || !tile_ok || !lds_ok ||
-      t->rects > 2) {
+      t->rects > 2 || t->colors > 2) {
     gt_set_error("gt_scene_set_tuning: value out of range (path %u, refill %u, comp %u, blocks %u, tile_w %u, "
-                 "stack_lds %u, rects %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu, t->tile_w,
-                 t->stack_lds, t->rects);
+                 "stack_lds %u, rects %u, colors %u)", t->path, t->refill_min, t->comp_min, t->blocks_per_cu,
+                 t->tile_w, t->stack_lds, t->rects, t->colors);
     return GT_E_INVALID;
   }
   s->tune = *t;

@@ -49,6 +49,9 @@ struct gt_scene {
   /* the proxy rectangles of one-camera frames, one table per stream slot
    * of ctl (sized on the stream's first render after a rebuild) */
   DevBuf rects[GT_WORK_SLOTS];
+  /* the per-Gaussian colours of a stream's current frame, one per stream
+   * slot of ctl (sized on the stream's first render after a rebuild) */
+  DevBuf colors[GT_WORK_SLOTS];
   DevBuf bwd; /* backward scratch: per-Gaussian accumulators and colours */
   DevBuf rc;  /* recompute-mode scratch: band checkpoint, counts, image */
   DevBuf aa;  /* antialiasing: the new frame before it is averaged in */

@@ -75,7 +75,7 @@ class SceneInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_uint32), ("refill_min", C.c_uint32), ("comp_min", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("tile_w", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("rects", C.c_uint32)]
+                ("rects", C.c_uint32), ("colors", C.c_uint32)]
 
 
 class AutotuneResult(C.Structure):
@@ -349,13 +349,17 @@ class Scene:
         _check(_lib.gt_scene_refit_device(self._h, raw_dev, stream_ptr, C.byref(rebuilt)))
         return bool(rebuilt.value)
 
-    def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0, rects=0):
+    def set_tuning(self, path=0, refill_min=0, comp_min=0, blocks_per_cu=0, tile_w=0, stack_lds=0, rects=0,
+                   colors=0):
         """Launch tuning (results do not depend on it); 0 = library default.
         rects: screen-rectangle culling of one-camera frames, 0 when the frame
         has at least a quarter as many rays as the scene has proxies, 1 always,
-        2 never."""
+        2 never.  colors: one-camera renders composite per-Gaussian colours
+        computed for the frame's origin before the traversal, 0 when the frame
+        has at least half as many rays as the scene has Gaussians, 1 always,
+        2 never (each hit evaluates its SH record)."""
         _check(_lib.gt_scene_set_tuning(self._h, C.byref(Tuning(path, refill_min, comp_min, blocks_per_cu, tile_w,
-                                                                stack_lds, rects))))
+                                                                stack_lds, rects, colors))))
 
     def get_tuning(self):
         t = Tuning()
